@@ -1,0 +1,410 @@
+"""accfft_amd -- MI355X (gfx950) implementation of AccFFT's single-GPU hot path.
+
+Python mirror of the reference's GPU C++ API (include/accfft_gpu.h:97-155,
+include/accfft_gpuf.h:97-158, include/accfft_operators_gpu.h:31-67) over the
+C ABI of ``libaccfft_amd.so`` (declared in include/accfft_amd.h).  Device
+memory is handled with torch tensors; every call is ordered on torch's
+current HIP stream.
+
+The native library is mandatory: if ``libaccfft_amd.so`` is missing or a GPU
+call fails, these functions raise -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Sequence
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libaccfft_amd.so")
+
+ACCFFT_FORWARD = -1
+ACCFFT_BACKWARD = +1
+ACCFFT_INVERSE = +1
+ACCFFT_ESTIMATE = 1
+ACCFFT_MEASURE = 2
+ACCFFT_PATIENT = 4
+BLOCKING = 0x100
+PRESERVE_INPUT = 0x200
+
+F64, F32 = 0, 1
+OK = 0
+
+GRAD_X, GRAD_Y, GRAD_Z, LAPLACE, BIHARMONIC, INV_LAPLACE, INV_BIHARMONIC = range(7)
+
+_lib = None
+_i3 = ctypes.c_int * 3
+_vp = ctypes.c_void_p
+
+
+class AccfftError(RuntimeError):
+    pass
+
+
+def lib() -> ctypes.CDLL:
+    """Loads libaccfft_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise AccfftError(
+            f"{LIB_PATH} is missing: build it with `python -m accfft_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    i64p = ctypes.POINTER(ctypes.c_int64)
+    dp = ctypes.POINTER(ctypes.c_double)
+    L.accfft_amd_status_string.restype = ctypes.c_char_p
+    L.accfft_amd_status_string.argtypes = [ctypes.c_int]
+    for nm in ("accfft_amd_local_size_dft_r2c", "accfft_amd_local_size_dft_c2c"):
+        getattr(L, nm).argtypes = [_i3] * 5 + [ctypes.c_int] * 5 + [i64p]
+    for nm in ("accfft_amd_plan_dft_3d_r2c", "accfft_amd_plan_dft_3d_c2c"):
+        getattr(L, nm).argtypes = [_i3, _vp, _vp, ctypes.c_int, ctypes.c_uint, ctypes.POINTER(_vp)]
+    L.accfft_amd_destroy_plan.argtypes = [_vp]
+    L.accfft_amd_plan_set_stream.argtypes = [_vp, _vp]
+    L.accfft_amd_plan_sync.argtypes = [_vp]
+    L.accfft_amd_plan_info.argtypes = [_vp, _i3, _i3, _i3, _i3, _i3, i64p,
+                                       ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                       ctypes.POINTER(ctypes.c_int)]
+    L.accfft_amd_execute_r2c.argtypes = [_vp, _vp, _vp, dp, ctypes.c_uint]
+    L.accfft_amd_execute_c2r.argtypes = [_vp, _vp, _vp, dp, ctypes.c_uint]
+    L.accfft_amd_execute_c2c.argtypes = [_vp, ctypes.c_int, _vp, _vp, dp, ctypes.c_uint]
+    L.accfft_amd_execute_r2c_axis.argtypes = [_vp, ctypes.c_int, _vp, _vp, dp]
+    L.accfft_amd_execute_c2r_axis.argtypes = [_vp, ctypes.c_int, _vp, _vp, dp]
+    L.accfft_amd_grad.argtypes = [_vp, _vp, _vp, _vp, _vp, ctypes.c_uint, dp, ctypes.c_int]
+    for nm in ("accfft_amd_laplace", "accfft_amd_biharmonic", "accfft_amd_inv_laplace",
+               "accfft_amd_inv_biharmonic"):
+        getattr(L, nm).argtypes = [_vp, _vp, _vp, dp]
+    L.accfft_amd_divergence.argtypes = [_vp, _vp, _vp, _vp, _vp, dp, ctypes.c_int]
+    L.accfft_amd_mult_wave_number.argtypes = [ctypes.c_int, ctypes.c_int, _vp, _vp, _i3, _i3,
+                                              _i3, ctypes.c_uint, _vp]
+    L.accfft_amd_axpy.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_double, _vp, _vp, _vp]
+    L.accfft_amd_local_transpose.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64, _vp,
+                                             _vp, ctypes.c_int, _vp]
+    for nm in ("accfft_amd_local_transpose_ragged", "accfft_amd_local_transpose_col"):
+        getattr(L, nm).argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.c_int64,
+                                   _vp, _vp, ctypes.c_int, _vp]
+    L.accfft_amd_local_transpose_inplace.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64,
+                                                     _vp, ctypes.c_int, _vp]
+    L.accfft_amd_last_elapsed_ms.argtypes = [_vp, ctypes.POINTER(ctypes.c_float)]
+    _lib = L
+    return L
+
+
+def _check(status: int, what: str) -> None:
+    if status != OK:
+        msg = lib().accfft_amd_status_string(status).decode()
+        raise AccfftError(f"{what}: {msg} (status {status})")
+
+
+def _stream():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t) -> Optional[int]:
+    if t is None:
+        return None
+    return t.data_ptr()
+
+
+def _fp_of(dtype) -> int:
+    import torch
+    if dtype in (torch.float32, torch.complex64):
+        return F32
+    if dtype in (torch.float64, torch.complex128):
+        return F64
+    raise AccfftError(f"unsupported dtype {dtype}")
+
+
+# ------------------------------------------------------------------ sizes
+def local_size_dft_r2c(n: Sequence[int], precision: int = F64, grid=(1, 1), coords=(0, 0)):
+    """accfft_local_size_dft_r2c_gpu(f) (src/accfft_gpu.cpp:62-65).
+    Returns (alloc_max_bytes, isize, istart, osize, ostart)."""
+    arr = [_i3() for _ in range(4)]
+    amax = ctypes.c_int64()
+    _check(lib().accfft_amd_local_size_dft_r2c(_i3(*n), *arr, precision, grid[0], grid[1],
+                                                 coords[0], coords[1], ctypes.byref(amax)),
+           "local_size_dft_r2c")
+    return (amax.value,) + tuple(tuple(a) for a in arr)
+
+
+def local_size_dft_c2c(n: Sequence[int], precision: int = F64, grid=(1, 1), coords=(0, 0)):
+    """accfft_local_size_dft_c2c_gpu(f) (src/accfft_gpu.cpp:689-692)."""
+    arr = [_i3() for _ in range(4)]
+    amax = ctypes.c_int64()
+    _check(lib().accfft_amd_local_size_dft_c2c(_i3(*n), *arr, precision, grid[0], grid[1],
+                                                 coords[0], coords[1], ctypes.byref(amax)),
+           "local_size_dft_c2c")
+    return (amax.value,) + tuple(tuple(a) for a in arr)
+
+
+# ------------------------------------------------------------------ plan
+class Plan:
+    """accfft_plan_gpu(f) on a 1x1 grid (include/accfft_gpu.h:37-95)."""
+
+    def __init__(self, kind: str, n: Sequence[int], data=None, data_out=None,
+                 precision: int = F64, flags: int = ACCFFT_MEASURE):
+        self.kind = kind
+        self.n = tuple(int(v) for v in n)
+        self.precision = precision
+        h = _vp()
+        fn = lib().accfft_amd_plan_dft_3d_r2c if kind == "r2c" else lib().accfft_amd_plan_dft_3d_c2c
+        _check(fn(_i3(*self.n), _ptr(data), _ptr(data_out), precision, flags, ctypes.byref(h)),
+               f"plan_dft_3d_{kind}")
+        self._h = h
+        isz, ist, osz, ost = _i3(), _i3(), _i3(), _i3()
+        amax = ctypes.c_int64()
+        fp, knd, inp = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().accfft_amd_plan_info(h, _i3(), isz, ist, osz, ost, ctypes.byref(amax),
+                                          ctypes.byref(fp), ctypes.byref(knd), ctypes.byref(inp)),
+               "plan_info")
+        self.isize, self.istart = tuple(isz), tuple(ist)
+        self.osize, self.ostart = tuple(osz), tuple(ost)
+        self.alloc_max = amax.value
+        self.inplace = bool(inp.value)
+        self._keep = (data, data_out)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def destroy(self):
+        if self._h:
+            lib().accfft_amd_destroy_plan(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def _bind(self):
+        if not self._h:
+            raise AccfftError("plan destroyed")
+        _check(lib().accfft_amd_plan_set_stream(self._h, _stream()), "set_stream")
+        return self._h
+
+    @staticmethod
+    def _timer(timer):
+        if timer is None:
+            return None, None
+        arr = (ctypes.c_double * len(timer))(*timer)
+        return arr, ctypes.cast(arr, ctypes.POINTER(ctypes.c_double))
+
+    @staticmethod
+    def _untimer(timer, arr):
+        if timer is not None:
+            for i in range(len(timer)):
+                timer[i] = arr[i]
+
+    def execute_r2c(self, data=None, data_out=None, timer=None, xyz: int = 7):
+        arr, tp = self._timer(timer)
+        _check(lib().accfft_amd_execute_r2c(self._bind(), _ptr(data), _ptr(data_out), tp, xyz),
+               "execute_r2c")
+        self._untimer(timer, arr)
+
+    def execute_c2r(self, data=None, data_out=None, timer=None, xyz: int = 7):
+        arr, tp = self._timer(timer)
+        _check(lib().accfft_amd_execute_c2r(self._bind(), _ptr(data), _ptr(data_out), tp, xyz),
+               "execute_c2r")
+        self._untimer(timer, arr)
+
+    def execute_c2c(self, direction: int, data=None, data_out=None, timer=None, xyz: int = 7):
+        arr, tp = self._timer(timer)
+        _check(lib().accfft_amd_execute_c2c(self._bind(), direction, _ptr(data), _ptr(data_out),
+                                            tp, xyz), "execute_c2c")
+        self._untimer(timer, arr)
+
+    def execute_r2c_axis(self, axis: int, data, data_out, timer=None):
+        arr, tp = self._timer(timer)
+        _check(lib().accfft_amd_execute_r2c_axis(self._bind(), axis, _ptr(data), _ptr(data_out),
+                                                 tp), "execute_r2c_axis")
+        self._untimer(timer, arr)
+
+    def execute_c2r_axis(self, axis: int, data, data_out, timer=None):
+        arr, tp = self._timer(timer)
+        _check(lib().accfft_amd_execute_c2r_axis(self._bind(), axis, _ptr(data), _ptr(data_out),
+                                                 tp), "execute_c2r_axis")
+        self._untimer(timer, arr)
+
+    # operators (include/accfft_operators_gpu.h:44-67)
+    def grad(self, A, Ax=None, Ay=None, Az=None, mask: int = 7, timer=None, slow=False):
+        arr, tp = self._timer(timer)
+        _check(lib().accfft_amd_grad(self._bind(), _ptr(Ax), _ptr(Ay), _ptr(Az), _ptr(A), mask,
+                                     tp, int(slow)), "grad")
+        self._untimer(timer, arr)
+
+    def _spectral(self, name, out, A, timer):
+        arr, tp = self._timer(timer)
+        _check(getattr(lib(), name)(self._bind(), _ptr(out), _ptr(A), tp), name)
+        self._untimer(timer, arr)
+
+    def laplace(self, A, out, timer=None):
+        self._spectral("accfft_amd_laplace", out, A, timer)
+
+    def biharmonic(self, A, out, timer=None):
+        self._spectral("accfft_amd_biharmonic", out, A, timer)
+
+    def inv_laplace(self, A, out, timer=None):
+        self._spectral("accfft_amd_inv_laplace", out, A, timer)
+
+    def inv_biharmonic(self, A, out, timer=None):
+        self._spectral("accfft_amd_inv_biharmonic", out, A, timer)
+
+    def divergence(self, Ax, Ay, Az, out, timer=None, slow=False):
+        arr, tp = self._timer(timer)
+        _check(lib().accfft_amd_divergence(self._bind(), _ptr(out), _ptr(Ax), _ptr(Ay), _ptr(Az),
+                                           tp, int(slow)), "divergence")
+        self._untimer(timer, arr)
+
+    def sync(self):
+        _check(lib().accfft_amd_plan_sync(self._bind()), "sync")
+
+    def last_elapsed_ms(self) -> float:
+        v = ctypes.c_float()
+        _check(lib().accfft_amd_last_elapsed_ms(self._h, ctypes.byref(v)), "last_elapsed_ms")
+        return v.value
+
+
+# ------------------------------------------------------------------ reference-named API
+def accfft_local_size_dft_r2c_gpu(n):
+    """FP64 sizes (src/accfft_gpu.cpp:62-65); returns (alloc_max, isize, istart, osize, ostart)."""
+    return local_size_dft_r2c(n, F64)
+
+
+def accfft_local_size_dft_r2c_gpuf(n):
+    return local_size_dft_r2c(n, F32)
+
+
+def accfft_local_size_dft_c2c_gpu(n):
+    return local_size_dft_c2c(n, F64)
+
+
+def accfft_local_size_dft_c2c_gpuf(n):
+    return local_size_dft_c2c(n, F32)
+
+
+def accfft_plan_dft_3d_r2c_gpu(n, data, data_out, flags=ACCFFT_MEASURE) -> Plan:
+    return Plan("r2c", n, data, data_out, F64, flags)
+
+
+def accfft_plan_dft_3d_r2c_gpuf(n, data, data_out, flags=ACCFFT_MEASURE) -> Plan:
+    return Plan("r2c", n, data, data_out, F32, flags)
+
+
+def accfft_plan_dft_3d_c2c_gpu(n, data, data_out, flags=ACCFFT_MEASURE) -> Plan:
+    return Plan("c2c", n, data, data_out, F64, flags)
+
+
+def accfft_plan_dft_3d_c2c_gpuf(n, data, data_out, flags=ACCFFT_MEASURE) -> Plan:
+    return Plan("c2c", n, data, data_out, F32, flags)
+
+
+def accfft_execute_r2c_gpu(plan: Plan, data=None, data_out=None, timer=None, xyz=7):
+    plan.execute_r2c(data, data_out, timer, xyz)
+
+
+accfft_execute_r2c_gpuf = accfft_execute_r2c_gpu
+
+
+def accfft_execute_c2r_gpu(plan: Plan, data=None, data_out=None, timer=None, xyz=7):
+    plan.execute_c2r(data, data_out, timer, xyz)
+
+
+accfft_execute_c2r_gpuf = accfft_execute_c2r_gpu
+
+
+def accfft_execute_c2c_gpu(plan: Plan, direction, data=None, data_out=None, timer=None, xyz=7):
+    plan.execute_c2c(direction, data, data_out, timer, xyz)
+
+
+accfft_execute_c2c_gpuf = accfft_execute_c2c_gpu
+
+
+def accfft_destroy_plan_gpu(plan: Plan):
+    plan.destroy()
+
+
+accfft_destroy_plan = accfft_destroy_plan_gpu
+
+
+def accfft_cleanup_gpu():
+    """src/accfft_gpu.cpp:45-47 (nothing to release)."""
+
+
+accfft_cleanup_gpuf = accfft_cleanup_gpu
+
+
+def accfft_grad_gpu(A_x, A_y, A_z, A, plan: Plan, pXYZ: int = 7, timer=None):
+    plan.grad(A, A_x, A_y, A_z, pXYZ, timer)
+
+
+accfft_grad_gpuf = accfft_grad_gpu
+
+
+def accfft_laplace_gpu(LA, A, plan: Plan, timer=None):
+    plan.laplace(A, LA, timer)
+
+
+accfft_laplace_gpuf = accfft_laplace_gpu
+
+
+def accfft_biharmonic_gpu(BA, A, plan: Plan, timer=None):
+    plan.biharmonic(A, BA, timer)
+
+
+accfft_biharmonic_gpuf = accfft_biharmonic_gpu
+
+
+def accfft_divergence_gpu(divA, A_x, A_y, A_z, plan: Plan, timer=None):
+    plan.divergence(A_x, A_y, A_z, divA, timer)
+
+
+accfft_divergence_gpuf = accfft_divergence_gpu
+
+
+def accfft_inv_laplace_gpu(invLA, A, plan: Plan, timer=None):
+    plan.inv_laplace(A, invLA, timer)
+
+
+accfft_inv_laplace_gpuf = accfft_inv_laplace_gpu
+
+
+# ------------------------------------------------------------------ element-wise / transposes
+def mult_wave_number(kind: int, wA, A, n, osize, ostart, scale_xyz: int = 7):
+    """The extern "C" multipliers of src/operators_cuda.cu:421-486."""
+    _check(lib().accfft_amd_mult_wave_number(kind, _fp_of(A.dtype), _ptr(wA), _ptr(A), _i3(*n),
+                                             _i3(*osize), _i3(*ostart), scale_xyz, _stream()),
+           "mult_wave_number")
+
+
+def axpy(alpha: float, x, y):
+    """daxpy_gpu_c(f), src/operators_cuda.cu:470-474: y += alpha * x."""
+    _check(lib().accfft_amd_axpy(_fp_of(x.dtype), x.numel(), alpha, _ptr(x), _ptr(y), _stream()),
+           "axpy")
+
+
+def local_transpose(r, c, t, inp, out):
+    """local_transpose_cuda(r,c,t,in,out), src/transpose_cuda.cu:242-257."""
+    _check(lib().accfft_amd_local_transpose(r, c, t, _ptr(inp), _ptr(out), inp.element_size(),
+                                            _stream()), "local_transpose")
+
+
+def local_transpose_ragged(r, c, t, t2, inp, out):
+    """local_transpose_cuda(r,c,t,t2,in,out), src/transpose_cuda.cu:207-224."""
+    _check(lib().accfft_amd_local_transpose_ragged(r, c, t, t2, _ptr(inp), _ptr(out),
+                                                   inp.element_size(), _stream()),
+           "local_transpose_ragged")
+
+
+def local_transpose_col(r, c, t, t2, inp, out):
+    """local_transpose_col_cuda(r,c,t,t2,in,out), src/transpose_cuda.cu:78-95."""
+    _check(lib().accfft_amd_local_transpose_col(r, c, t, t2, _ptr(inp), _ptr(out),
+                                                inp.element_size(), _stream()),
+           "local_transpose_col")
+
+
+def local_transpose_inplace(r, c, t, A):
+    """local_transpose_cuda(r,c,t,A), src/transpose_cuda.cu:227-239."""
+    _check(lib().accfft_amd_local_transpose_inplace(r, c, t, _ptr(A), A.element_size(),
+                                                    _stream()), "local_transpose_inplace")

@@ -1,0 +1,407 @@
+// capi.hip -- the extern "C" boundary declared in include/accfft_amd.h.
+//
+// Plan creation restates accfft_plan_dft_3d_{r2c,c2c}_gpu
+// (src/accfft_gpu.cpp:77-442, :704-931) for a 1x1 grid: sizes come from the
+// same local-size rules (src/accfft_common.cpp:49-256, kept in int64 here),
+// in-place is decided by pointer equality (:99-103), and the cuFFT plans are
+// replaced by device twiddle tables per axis length.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cmath>
+#include <new>
+
+#include "../../include/accfft_amd.h"
+#include "dispatch.hpp"
+#include "plan.hpp"
+
+namespace {
+
+// dfft_get_local_size_t (src/accfft_common.cpp:49-91); returns bytes.
+int64_t local_size(int N0, int N1, int tuple, int* isize, int* istart, int p0, int p1, int c0,
+                   int c1, int elem) {
+  isize[2] = tuple;
+  isize[0] = (int)std::ceil(N0 / (double)p0);
+  isize[1] = (int)std::ceil(N1 / (double)p1);
+  istart[0] = isize[0] * c0;
+  istart[1] = isize[1] * c1;
+  istart[2] = 0;
+  if ((N0 - isize[0] * c0) < isize[0]) {
+    isize[0] = N0 - isize[0] * c0;
+    isize[0] *= (int)(isize[0] > 0);
+    istart[0] = N0 - isize[0];
+  }
+  if ((N1 - isize[1] * c1) < isize[1]) {
+    isize[1] = N1 - isize[1] * c1;
+    isize[1] *= (int)(isize[1] > 0);
+    istart[1] = N1 - isize[1];
+  }
+  return (int64_t)isize[0] * isize[1] * isize[2] * elem;
+}
+
+bool bad_grid(const int n[3], int p0, int p1, int c0, int c1) {
+  if (!n) return true;
+  for (int i = 0; i < 3; ++i)
+    if (n[i] <= 0) return true;
+  return p0 <= 0 || p1 <= 0 || c0 < 0 || c1 < 0 || c0 >= p0 || c1 >= p1;
+}
+
+// accfft_local_size_dft_r2c_t (src/accfft_common.cpp:98-192)
+int64_t size_r2c(const int* n, int* isize, int* istart, int* osize, int* ostart, int p0, int p1,
+                 int c0, int c1, int elem) {
+  int o0[3], s0[3], o1[3], s1[3], o2[3], s2[3], oy[3], sy[3], oyi[3], syi[3], ox[3], sx[3],
+      oxi[3], sxi[3];
+  const int nt = (n[2] / 2 + 1) * 2;
+  int64_t amax = 0;
+  amax = std::max(amax, local_size(n[0], n[1], nt, o0, s0, p0, p1, c0, c1, elem));
+  amax = std::max(amax, 2 * local_size(n[0], nt / 2, n[1], o1, s1, p0, p1, c0, c1, elem));
+  amax = std::max(amax, 2 * local_size(n[1], nt / 2, n[0], o2, s2, p0, p1, c0, c1, elem));
+  std::swap(o2[1], o2[2]);
+  std::swap(o2[0], o2[1]);
+  std::swap(s2[1], s2[2]);
+  std::swap(s2[0], s2[1]);
+  local_size(n[0], n[1], n[2], isize, istart, p0, p1, c0, c1, elem);
+  for (int i = 0; i < 3; ++i) {
+    osize[i] = o2[i];
+    ostart[i] = s2[i];
+  }
+  amax = std::max(amax, local_size(n[0], n[2], n[1], oy, sy, p0, p1, c0, c1, elem));
+  amax = std::max(amax, 2 * local_size(n[0], n[2], n[1] / 2 + 1, oyi, syi, p0, p1, c0, c1, elem));
+  local_size(isize[1] * n[2], n[2], n[0], ox, sx, p0, p1, c0, c1, elem);
+  amax = std::max(amax, (int64_t)ox[2] * ox[0] * 1 * elem);
+  local_size(isize[1] * n[2], n[2], n[0] / 2 + 1, oxi, sxi, p0, p1, c0, c1, elem);
+  amax = std::max(amax, 2 * (int64_t)oxi[2] * oxi[0] * 1 * elem);
+  return amax;
+}
+
+// accfft_local_size_dft_c2c_t (src/accfft_common.cpp:203-256)
+int64_t size_c2c(const int* n, int* isize, int* istart, int* osize, int* ostart, int p0, int p1,
+                 int c0, int c1, int elem) {
+  int o0[3], s0[3], o1[3], s1[3], o2[3], s2[3];
+  int64_t amax = 0;
+  amax = std::max(amax, local_size(n[0], n[1], n[2], o0, s0, p0, p1, c0, c1, elem));
+  amax = std::max(amax, local_size(n[0], n[2], n[1], o1, s1, p0, p1, c0, c1, elem));
+  amax = std::max(amax, local_size(n[1], n[2], n[0], o2, s2, p0, p1, c0, c1, elem));
+  amax *= 2;
+  std::swap(s2[1], s2[2]);
+  std::swap(s2[0], s2[1]);
+  std::swap(o2[1], o2[2]);
+  std::swap(o2[0], o2[1]);
+  local_size(n[0], n[1], n[2], isize, istart, p0, p1, c0, c1, elem);
+  for (int i = 0; i < 3; ++i) {
+    osize[i] = o2[i];
+    ostart[i] = s2[i];
+  }
+  return amax;
+}
+
+int elem_of(int fp) { return fp == ACCFFT_AMD_F32 ? 4 : 8; }
+
+void free_plan(accfft_amd_plan* p) {
+  if (!p) return;
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
+  for (void* m : p->dev_mem) (void)hipFree(m);
+  for (int i = 0; i < 2; ++i)
+    if (p->ws[i]) (void)hipFree(p->ws[i]);
+  if (p->ev0) (void)hipEventDestroy(p->ev0);
+  if (p->ev1) (void)hipEventDestroy(p->ev1);
+  delete p;
+}
+
+int make_plan(int kind, const int n[3], void* data, void* data_out, int fp, unsigned flags,
+              accfft_amd_plan** out) {
+  if (!out) return ACCFFT_AMD_ERR_NULL;
+  *out = nullptr;
+  if (bad_grid(n, 1, 1, 0, 0)) return ACCFFT_AMD_ERR_INVALID;
+  if (fp != ACCFFT_AMD_F32 && fp != ACCFFT_AMD_F64) return ACCFFT_AMD_ERR_INVALID;
+  // every length must fit one LDS tile of the dynamic engine
+  for (int i = 0; i < 3; ++i) {
+    const size_t csz = fp == ACCFFT_AMD_F32 ? 8 : 16;
+    const int nn = n[i];
+    if (!afft::static_length(nn) && afft::dyn_z_lds(nn, afft::dyn_lines(nn), csz) > 160 * 1024)
+      return ACCFFT_AMD_ERR_UNSUPPORTED;
+  }
+  accfft_amd_plan* p = new (std::nothrow) accfft_amd_plan();
+  if (!p) return ACCFFT_AMD_ERR_ALLOC;
+  for (int i = 0; i < 3; ++i) p->n[i] = n[i];
+  p->fp = fp;
+  p->kind = kind;
+  p->flags = flags;
+  p->data = data;
+  p->data_out = data_out;
+  p->inplace = (data == data_out);
+  const int elem = elem_of(fp);
+  p->alloc_max = kind == 1 ? size_r2c(n, p->isize, p->istart, p->osize, p->ostart, 1, 1, 0, 0, elem)
+                           : size_c2c(n, p->isize, p->istart, p->osize, p->ostart, 1, 1, 0, 0, elem);
+  (void)hipGetDevice(&p->device);
+  if (hipEventCreate(&p->ev0) != hipSuccess || hipEventCreate(&p->ev1) != hipSuccess) {
+    free_plan(p);
+    return ACCFFT_AMD_ERR_HIP;
+  }
+  const int st = fp == ACCFFT_AMD_F32 ? afft_tables_f32(p) : afft_tables_f64(p);
+  if (st) {
+    free_plan(p);
+    return st;
+  }
+  *out = p;
+  return ACCFFT_AMD_OK;
+}
+
+// Wraps one execute: events around the launches, optional timer / blocking.
+template <typename F>
+int timed(accfft_amd_plan* p, double* timer, F&& body) {
+  (void)hipEventRecord(p->ev0, p->stream);
+  int st = body();
+  (void)hipEventRecord(p->ev1, p->stream);
+  if (st) return st;
+  if (timer || (p->flags & ACCFFT_AMD_BLOCKING)) {
+    if (hipEventSynchronize(p->ev1) != hipSuccess) return ACCFFT_AMD_ERR_HIP;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, p->ev0, p->ev1);
+    p->last_ms = ms;
+    if (timer) timer[4] += ms / 1000.0;
+  }
+  return ACCFFT_AMD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* accfft_amd_status_string(int s) {
+  switch (s) {
+    case ACCFFT_AMD_OK: return "ok";
+    case ACCFFT_AMD_ERR_INVALID: return "invalid argument";
+    case ACCFFT_AMD_ERR_OVERFLOW: return "size overflow";
+    case ACCFFT_AMD_ERR_UNSUPPORTED: return "unsupported transform length";
+    case ACCFFT_AMD_ERR_NOT_BAKED: return "plan of the wrong kind for this call";
+    case ACCFFT_AMD_ERR_HIP: return "HIP runtime error";
+    case ACCFFT_AMD_ERR_ALLOC: return "device allocation failed";
+    case ACCFFT_AMD_ERR_NULL: return "NULL plan or pointer";
+    default: return "unknown status";
+  }
+}
+
+int accfft_amd_version(void) { return 100; }
+
+int accfft_amd_local_size_dft_r2c(const int n[3], int isize[3], int istart[3], int osize[3],
+                                  int ostart[3], int fp, int p0, int p1, int c0, int c1,
+                                  int64_t* alloc_max) {
+  if (!isize || !istart || !osize || !ostart) return ACCFFT_AMD_ERR_NULL;
+  if (bad_grid(n, p0, p1, c0, c1)) return ACCFFT_AMD_ERR_INVALID;
+  if (fp != ACCFFT_AMD_F32 && fp != ACCFFT_AMD_F64) return ACCFFT_AMD_ERR_INVALID;
+  const int64_t a = size_r2c(n, isize, istart, osize, ostart, p0, p1, c0, c1, elem_of(fp));
+  if (alloc_max) *alloc_max = a;
+  return ACCFFT_AMD_OK;
+}
+
+int accfft_amd_local_size_dft_c2c(const int n[3], int isize[3], int istart[3], int osize[3],
+                                  int ostart[3], int fp, int p0, int p1, int c0, int c1,
+                                  int64_t* alloc_max) {
+  if (!isize || !istart || !osize || !ostart) return ACCFFT_AMD_ERR_NULL;
+  if (bad_grid(n, p0, p1, c0, c1)) return ACCFFT_AMD_ERR_INVALID;
+  if (fp != ACCFFT_AMD_F32 && fp != ACCFFT_AMD_F64) return ACCFFT_AMD_ERR_INVALID;
+  const int64_t a = size_c2c(n, isize, istart, osize, ostart, p0, p1, c0, c1, elem_of(fp));
+  if (alloc_max) *alloc_max = a;
+  return ACCFFT_AMD_OK;
+}
+
+int accfft_amd_plan_dft_3d_r2c(const int n[3], void* data, void* data_out, int fp,
+                               unsigned flags, accfft_amd_plan** plan) {
+  return make_plan(1, n, data, data_out, fp, flags, plan);
+}
+
+int accfft_amd_plan_dft_3d_c2c(const int n[3], void* data, void* data_out, int fp,
+                               unsigned flags, accfft_amd_plan** plan) {
+  return make_plan(2, n, data, data_out, fp, flags, plan);
+}
+
+int accfft_amd_destroy_plan(accfft_amd_plan* plan) {
+  if (!plan) return ACCFFT_AMD_ERR_NULL;
+  free_plan(plan);
+  return ACCFFT_AMD_OK;
+}
+
+int accfft_amd_plan_set_stream(accfft_amd_plan* plan, void* stream) {
+  if (!plan) return ACCFFT_AMD_ERR_NULL;
+  plan->stream = (hipStream_t)stream;
+  return ACCFFT_AMD_OK;
+}
+
+int accfft_amd_plan_info(const accfft_amd_plan* p, int n[3], int isize[3], int istart[3],
+                         int osize[3], int ostart[3], int64_t* alloc_max, int* fp, int* kind,
+                         int* inplace) {
+  if (!p) return ACCFFT_AMD_ERR_NULL;
+  for (int i = 0; i < 3; ++i) {
+    if (n) n[i] = p->n[i];
+    if (isize) isize[i] = p->isize[i];
+    if (istart) istart[i] = p->istart[i];
+    if (osize) osize[i] = p->osize[i];
+    if (ostart) ostart[i] = p->ostart[i];
+  }
+  if (alloc_max) *alloc_max = p->alloc_max;
+  if (fp) *fp = p->fp;
+  if (kind) *kind = p->kind;
+  if (inplace) *inplace = p->inplace ? 1 : 0;
+  return ACCFFT_AMD_OK;
+}
+
+int accfft_amd_plan_data(const accfft_amd_plan* p, void** data, void** data_out) {
+  if (!p) return ACCFFT_AMD_ERR_NULL;
+  if (data) *data = p->data;
+  if (data_out) *data_out = p->data_out;
+  return ACCFFT_AMD_OK;
+}
+
+int accfft_amd_execute_r2c(accfft_amd_plan* p, const void* in, void* out, double* timer,
+                           unsigned xyz) {
+  if (!p) return ACCFFT_AMD_ERR_NULL;
+  if (p->kind != 1) return ACCFFT_AMD_ERR_NOT_BAKED;
+  if (!in) in = p->data;
+  if (!out) out = p->data_out;
+  if (!in || !out) return ACCFFT_AMD_ERR_NULL;
+  xyz &= 7u;
+  return timed(p, timer, [&] {
+    return p->fp == ACCFFT_AMD_F32 ? afft_r2c_f32(p, in, out, xyz) : afft_r2c_f64(p, in, out, xyz);
+  });
+}
+
+int accfft_amd_execute_c2r(accfft_amd_plan* p, void* in, void* out, double* timer,
+                           unsigned xyz) {
+  if (!p) return ACCFFT_AMD_ERR_NULL;
+  if (p->kind != 1) return ACCFFT_AMD_ERR_NOT_BAKED;
+  if (!in) in = p->data_out;
+  if (!out) out = p->data;
+  if (!in || !out) return ACCFFT_AMD_ERR_NULL;
+  xyz &= 7u;
+  return timed(p, timer, [&] {
+    return p->fp == ACCFFT_AMD_F32 ? afft_c2r_f32(p, in, out, xyz) : afft_c2r_f64(p, in, out, xyz);
+  });
+}
+
+int accfft_amd_execute_c2c(accfft_amd_plan* p, int direction, void* in, void* out,
+                           double* timer, unsigned xyz) {
+  if (!p) return ACCFFT_AMD_ERR_NULL;
+  if (p->kind != 2) return ACCFFT_AMD_ERR_NOT_BAKED;
+  if (direction != -1 && direction != 1) return ACCFFT_AMD_ERR_INVALID;
+  if (!in) in = p->data;
+  if (!out) out = p->data_out;
+  if (!in || !out) return ACCFFT_AMD_ERR_NULL;
+  xyz &= 7u;
+  return timed(p, timer, [&] {
+    return p->fp == ACCFFT_AMD_F32 ? afft_c2c_f32(p, direction, in, out, xyz)
+                                   : afft_c2c_f64(p, direction, in, out, xyz);
+  });
+}
+
+int accfft_amd_execute_r2c_axis(accfft_amd_plan* p, int axis, const void* in, void* out,
+                                double* timer) {
+  if (!p) return ACCFFT_AMD_ERR_NULL;
+  if (p->kind != 1) return ACCFFT_AMD_ERR_NOT_BAKED;
+  if (axis < 0 || axis > 2) return ACCFFT_AMD_ERR_INVALID;
+  if (!in) in = p->data;
+  if (!out) out = p->data_out;
+  if (!in || !out) return ACCFFT_AMD_ERR_NULL;
+  return timed(p, timer, [&] {
+    return p->fp == ACCFFT_AMD_F32 ? afft_r2c_axis_f32(p, axis, in, out)
+                                   : afft_r2c_axis_f64(p, axis, in, out);
+  });
+}
+
+int accfft_amd_execute_c2r_axis(accfft_amd_plan* p, int axis, const void* in, void* out,
+                                double* timer) {
+  if (!p) return ACCFFT_AMD_ERR_NULL;
+  if (p->kind != 1) return ACCFFT_AMD_ERR_NOT_BAKED;
+  if (axis < 0 || axis > 2) return ACCFFT_AMD_ERR_INVALID;
+  if (!in) in = p->data_out;
+  if (!out) out = p->data;
+  if (!in || !out) return ACCFFT_AMD_ERR_NULL;
+  return timed(p, timer, [&] {
+    return p->fp == ACCFFT_AMD_F32 ? afft_c2r_axis_f32(p, axis, in, out)
+                                   : afft_c2r_axis_f64(p, axis, in, out);
+  });
+}
+
+int accfft_amd_grad(accfft_amd_plan* p, void* Ax, void* Ay, void* Az, const void* A,
+                    unsigned mask, double* timer, int slow) {
+  if (!p || !A) return ACCFFT_AMD_ERR_NULL;
+  if (p->kind != 1) return ACCFFT_AMD_ERR_NOT_BAKED;
+  return timed(p, timer, [&] {
+    return p->fp == ACCFFT_AMD_F32 ? afft_grad_f32(p, Ax, Ay, Az, A, mask, slow)
+                                   : afft_grad_f64(p, Ax, Ay, Az, A, mask, slow);
+  });
+}
+
+static int spectral(accfft_amd_plan* p, int sym, void* out, const void* A, double* timer) {
+  if (!p || !A || !out) return ACCFFT_AMD_ERR_NULL;
+  if (p->kind != 1) return ACCFFT_AMD_ERR_NOT_BAKED;
+  return timed(p, timer, [&] {
+    return p->fp == ACCFFT_AMD_F32 ? afft_spectral_f32(p, sym, out, A)
+                                   : afft_spectral_f64(p, sym, out, A);
+  });
+}
+
+int accfft_amd_laplace(accfft_amd_plan* p, void* LA, const void* A, double* timer) {
+  return spectral(p, afft::SYM_LAPLACE, LA, A, timer);
+}
+int accfft_amd_biharmonic(accfft_amd_plan* p, void* BA, const void* A, double* timer) {
+  return spectral(p, afft::SYM_BIHARMONIC, BA, A, timer);
+}
+int accfft_amd_inv_laplace(accfft_amd_plan* p, void* out, const void* A, double* timer) {
+  return spectral(p, afft::SYM_INV_LAPLACE, out, A, timer);
+}
+int accfft_amd_inv_biharmonic(accfft_amd_plan* p, void* out, const void* A, double* timer) {
+  return spectral(p, afft::SYM_INV_BIHARMONIC, out, A, timer);
+}
+
+int accfft_amd_divergence(accfft_amd_plan* p, void* divA, const void* Ax, const void* Ay,
+                          const void* Az, double* timer, int slow) {
+  if (!p || !divA || !Ax || !Ay || !Az) return ACCFFT_AMD_ERR_NULL;
+  if (p->kind != 1) return ACCFFT_AMD_ERR_NOT_BAKED;
+  return timed(p, timer, [&] {
+    return p->fp == ACCFFT_AMD_F32 ? afft_div_f32(p, divA, Ax, Ay, Az, slow)
+                                   : afft_div_f64(p, divA, Ax, Ay, Az, slow);
+  });
+}
+
+int accfft_amd_mult_wave_number(int kind, int fp, void* wA, const void* A, const int n[3],
+                                const int osize[3], const int ostart[3], unsigned scale_xyz,
+                                void* stream) {
+  if (!wA || !A || !n || !osize || !ostart) return ACCFFT_AMD_ERR_NULL;
+  if (kind < 0 || kind > 6) return ACCFFT_AMD_ERR_INVALID;
+  if (fp != ACCFFT_AMD_F32 && fp != ACCFFT_AMD_F64) return ACCFFT_AMD_ERR_INVALID;
+  for (int i = 0; i < 3; ++i)
+    if (n[i] <= 0 || osize[i] < 0) return ACCFFT_AMD_ERR_INVALID;
+  double scale = 1.0;
+  if (kind <= 2) {
+    for (int i = 0; i < 3; ++i)
+      if (scale_xyz & (1u << i)) scale *= n[i];
+  } else {
+    scale = (double)n[0] * n[1] * n[2];
+  }
+  scale = 1.0 / scale;
+  return fp == ACCFFT_AMD_F32 ? afft_mult_f32(kind, wA, A, n, osize, ostart, scale, stream)
+                              : afft_mult_f64(kind, wA, A, n, osize, ostart, scale, stream);
+}
+
+int accfft_amd_axpy(int fp, int64_t n, double alpha, const void* x, void* y, void* stream) {
+  if (!x || !y) return ACCFFT_AMD_ERR_NULL;
+  if (n < 0) return ACCFFT_AMD_ERR_INVALID;
+  return fp == ACCFFT_AMD_F32 ? afft_axpy_f32(n, alpha, x, y, stream)
+                              : afft_axpy_f64(n, alpha, x, y, stream);
+}
+
+int accfft_amd_plan_sync(accfft_amd_plan* p) {
+  if (!p) return ACCFFT_AMD_ERR_NULL;
+  return hipStreamSynchronize(p->stream) == hipSuccess ? ACCFFT_AMD_OK : ACCFFT_AMD_ERR_HIP;
+}
+
+int accfft_amd_last_elapsed_ms(const accfft_amd_plan* p, float* ms) {
+  if (!p || !ms) return ACCFFT_AMD_ERR_NULL;
+  *ms = p->last_ms;
+  return ACCFFT_AMD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,318 @@
+// exec.hpp -- the execute paths of a plan, per precision T.
+//
+// Stage order follows src/accfft_gpu.cpp on a 1x1 grid:
+//   r2c (:471-539):  z R2C (in -> out), y C2C in place, x C2C in place
+//   c2r (:540-603):  x C2C, y C2C (in place on the input, which the
+//                    reference overwrites), z C2R (-> out)
+//   c2c (:943-1136): same order with a complex z pass
+// with the y pass issued as one launch instead of N0 (the reference loops
+// cufftExecZ2Z per x slab, :501-509).  The transposes between stages are
+// no-ops at P = 1 (src/transpose_gpu.cpp:3819-3821, :1164-1179) and are
+// not issued.
+#pragma once
+#include "plan.hpp"
+
+namespace afft {
+
+inline int status_of(hipError_t e) {
+  if (e == hipSuccess) return ACCFFT_AMD_OK;
+  if (e == hipErrorNotSupported) return ACCFFT_AMD_ERR_UNSUPPORTED;
+  if (e == hipErrorOutOfMemory) return ACCFFT_AMD_ERR_ALLOC;
+  return ACCFFT_AMD_ERR_HIP;
+}
+
+#define AFFT_TRY(x)                         \
+  do {                                      \
+    hipError_t e_ = (x);                    \
+    if (e_ != hipSuccess) return status_of(e_); \
+  } while (0)
+
+inline int ensure_ws(accfft_amd_plan* p, int i, size_t bytes) {
+  if (p->ws_bytes[i] >= bytes) return ACCFFT_AMD_OK;
+  if (p->ws[i]) {
+    (void)hipStreamSynchronize(p->stream);
+    (void)hipFree(p->ws[i]);
+    p->ws[i] = nullptr;
+    p->ws_bytes[i] = 0;
+  }
+  if (hipMalloc(&p->ws[i], bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return ACCFFT_AMD_ERR_ALLOC;
+  }
+  p->ws_bytes[i] = bytes;
+  return ACCFFT_AMD_OK;
+}
+
+template <typename T>
+struct Ops {
+  using C = cpx<T>;
+
+  static const AxisTab<T>& tab(const accfft_amd_plan* p, int ax);
+
+  static FftArgs<T> zargs(const void* in, void* out, int64_t nlines, int64_t idist,
+                          int64_t odist) {
+    FftArgs<T> a{};
+    a.in = in;
+    a.out = out;
+    a.nlines = nlines;
+    a.idist = idist;
+    a.odist = odist;
+    a.C = 1;
+    return a;
+  }
+  static FftArgs<T> sargs(const void* in, void* out, int64_t G, int64_t Cc, int64_t Din,
+                          int64_t Dout, int64_t Sin, int64_t Sout) {
+    FftArgs<T> a{};
+    a.in = in;
+    a.out = out;
+    a.nlines = G;
+    a.C = Cc;
+    a.Din = Din;
+    a.Dout = Dout;
+    a.Sin = Sin;
+    a.Sout = Sout;
+    return a;
+  }
+
+  // ---------------------------------------------------------- building blocks
+  // z pass of the r2c layout: real [N0][N1][N2 (or 2Nh)] <-> [N0][N1][Nh]
+  static int z_r2c(accfft_amd_plan* p, const void* in, void* out, bool padded) {
+    const int64_t N0 = p->n[0], N1 = p->n[1], N2 = p->n[2], Nh = N2 / 2 + 1;
+    auto a = zargs(in, out, N0 * N1, padded ? 2 * Nh : N2, Nh);
+    return status_of(launch_fft<T>(false, M_R2C, -1, a, tab(p, 2), p->stream));
+  }
+  static int z_c2r(accfft_amd_plan* p, const void* in, void* out, bool padded) {
+    const int64_t N0 = p->n[0], N1 = p->n[1], N2 = p->n[2], Nh = N2 / 2 + 1;
+    auto a = zargs(in, out, N0 * N1, Nh, padded ? 2 * Nh : N2);
+    return status_of(launch_fft<T>(false, M_C2R, +1, a, tab(p, 2), p->stream));
+  }
+  // y pass over a complex [N0][N1][L] array (L = Nh for r2c, N2 for c2c)
+  static FftArgs<T> y_args(const accfft_amd_plan* p, const void* in, void* out, int64_t L) {
+    const int64_t N0 = p->n[0], N1 = p->n[1];
+    return sargs(in, out, N0 * L, L, N1 * L, N1 * L, L, L);
+  }
+  // x pass over a complex [N0][N1][L] array
+  static FftArgs<T> x_args(const accfft_amd_plan* p, const void* in, void* out, int64_t L) {
+    const int64_t N1 = p->n[1];
+    return sargs(in, out, N1 * L, N1 * L, 0, 0, N1 * L, N1 * L);
+  }
+  static int y_c2c(accfft_amd_plan* p, const void* in, void* out, int64_t L, int sign) {
+    return status_of(launch_fft<T>(true, M_C2C, sign, y_args(p, in, out, L), tab(p, 1), p->stream));
+  }
+  static int x_c2c(accfft_amd_plan* p, const void* in, void* out, int64_t L, int sign) {
+    return status_of(launch_fft<T>(true, M_C2C, sign, x_args(p, in, out, L), tab(p, 0), p->stream));
+  }
+  // forward x, symbol, inverse x, fused (rows along x)
+  static int x_mult(accfft_amd_plan* p, const void* in, void* out, int64_t L, int sym,
+                    double scale) {
+    auto a = x_args(p, in, out, L);
+    a.sym = sym;
+    a.scale = scale;
+    a.N0 = p->n[0];
+    a.N1 = p->n[1];
+    a.N2 = p->n[2];
+    a.Ch = L;
+    a.maxis = 0;
+    return status_of(launch_fft<T>(true, M_MULT, -1, a, tab(p, 0), p->stream));
+  }
+  static int y_mult(accfft_amd_plan* p, const void* in, void* out, int64_t L, int sym,
+                    double scale) {
+    auto a = y_args(p, in, out, L);
+    a.sym = sym;
+    a.scale = scale;
+    a.N0 = p->n[0];
+    a.N1 = p->n[1];
+    a.N2 = p->n[2];
+    a.Ch = L;
+    a.maxis = 1;
+    return status_of(launch_fft<T>(true, M_MULT, -1, a, tab(p, 1), p->stream));
+  }
+  // fused per-axis derivative of a real field (operators_gpu.txx:172-278)
+  static int deriv(accfft_amd_plan* p, int axis, const void* in, void* out, int accumulate) {
+    const int64_t N0 = p->n[0], N1 = p->n[1], N2 = p->n[2];
+    FftArgs<T> a;
+    bool strided = true;
+    if (axis == 2) {
+      a = zargs(in, out, N0 * N1, N2, N2);
+      strided = false;
+    } else if (axis == 1) {
+      a = sargs(in, out, N0 * N2, N2, N1 * N2, N1 * N2, N2, N2);
+    } else {
+      a = sargs(in, out, N1 * N2, N1 * N2, 0, 0, N1 * N2, N1 * N2);
+    }
+    a.scale = 1.0 / (double)p->n[axis];
+    a.accumulate = accumulate;
+    return status_of(launch_fft<T>(strided, M_DERIV, -1, a, tab(p, axis), p->stream));
+  }
+
+  // ---------------------------------------------------------- public paths
+  static int r2c(accfft_amd_plan* p, const void* in, void* out, unsigned xyz) {
+    const int64_t Nh = p->n[2] / 2 + 1;
+    void* o = out;
+    int st;
+    if (xyz & 4) {
+      if ((st = z_r2c(p, in, out, p->inplace))) return st;
+    } else {
+      o = const_cast<void*>(in);  // src/accfft_gpu.cpp:488-489
+    }
+    if (xyz & 2)
+      if ((st = y_c2c(p, o, o, Nh, -1))) return st;
+    if (xyz & 1)
+      if ((st = x_c2c(p, o, o, Nh, -1))) return st;
+    return ACCFFT_AMD_OK;
+  }
+
+  static int c2r(accfft_amd_plan* p, void* in, void* out, unsigned xyz) {
+    const int64_t N0 = p->n[0], N1 = p->n[1], Nh = p->n[2] / 2 + 1;
+    const bool keep = (p->flags & ACCFFT_AMD_PRESERVE_INPUT) && (xyz & 4) && (xyz & 3);
+    void* cur = in;
+    int st;
+    if (keep) {
+      if ((st = ensure_ws(p, 0, (size_t)N0 * N1 * Nh * sizeof(C)))) return st;
+    }
+    if (xyz & 1) {
+      void* dst = keep ? p->ws[0] : cur;
+      if ((st = x_c2c(p, cur, dst, Nh, +1))) return st;
+      cur = dst;
+    }
+    if (xyz & 2) {
+      void* dst = (keep && cur == in) ? p->ws[0] : cur;
+      if ((st = y_c2c(p, cur, dst, Nh, +1))) return st;
+      cur = dst;
+    }
+    if (xyz & 4) return z_c2r(p, cur, out, p->inplace);
+    return ACCFFT_AMD_OK;  // result stays in the input (src/accfft_gpu.cpp:602-603)
+  }
+
+  static int c2c(accfft_amd_plan* p, int dir, void* in, void* out, unsigned xyz) {
+    const int64_t N0 = p->n[0], N1 = p->n[1], N2 = p->n[2];
+    int st;
+    if (dir < 0) {
+      void* o = out;
+      if (xyz & 4) {
+        auto a = zargs(in, out, N0 * N1, N2, N2);
+        if ((st = status_of(launch_fft<T>(false, M_C2C, -1, a, tab(p, 2), p->stream)))) return st;
+      } else {
+        o = in;
+      }
+      if (xyz & 2)
+        if ((st = y_c2c(p, o, o, N2, -1))) return st;
+      if (xyz & 1)
+        if ((st = x_c2c(p, o, o, N2, -1))) return st;
+      return ACCFFT_AMD_OK;
+    }
+    const bool keep = (p->flags & ACCFFT_AMD_PRESERVE_INPUT) && (xyz & 4) && (xyz & 3);
+    void* cur = in;
+    if (keep)
+      if ((st = ensure_ws(p, 0, (size_t)N0 * N1 * N2 * sizeof(C)))) return st;
+    if (xyz & 1) {
+      void* dst = keep ? p->ws[0] : cur;
+      if ((st = x_c2c(p, cur, dst, N2, +1))) return st;
+      cur = dst;
+    }
+    if (xyz & 2) {
+      void* dst = (keep && cur == in) ? p->ws[0] : cur;
+      if ((st = y_c2c(p, cur, dst, N2, +1))) return st;
+      cur = dst;
+    }
+    if (xyz & 4) {
+      auto a = zargs(cur, out, N0 * N1, N2, N2);
+      return status_of(launch_fft<T>(false, M_C2C, +1, a, tab(p, 2), p->stream));
+    }
+    return ACCFFT_AMD_OK;
+  }
+
+  // src/accfft_gpu.cpp:1212-1604
+  static int r2c_axis(accfft_amd_plan* p, int axis, const void* in, void* out) {
+    const int64_t N0 = p->n[0], N1 = p->n[1], N2 = p->n[2];
+    if (axis == 2) return z_r2c(p, in, out, false);
+    FftArgs<T> a;
+    if (axis == 1) a = sargs(in, out, N0 * N2, N2, N1 * N2, (N1 / 2 + 1) * N2, N2, N2);
+    else a = sargs(in, out, N1 * N2, N1 * N2, 0, 0, N1 * N2, N1 * N2);
+    return status_of(launch_fft<T>(true, M_R2C, -1, a, tab(p, axis), p->stream));
+  }
+  static int c2r_axis(accfft_amd_plan* p, int axis, const void* in, void* out) {
+    const int64_t N0 = p->n[0], N1 = p->n[1], N2 = p->n[2];
+    if (axis == 2) return z_c2r(p, in, out, false);
+    FftArgs<T> a;
+    if (axis == 1) a = sargs(in, out, N0 * N2, N2, (N1 / 2 + 1) * N2, N1 * N2, N2, N2);
+    else a = sargs(in, out, N1 * N2, N1 * N2, 0, 0, N1 * N2, N1 * N2);
+    return status_of(launch_fft<T>(true, M_C2R, +1, a, tab(p, axis), p->stream));
+  }
+
+  // ---------------------------------------------------------- operators
+  static int grad(accfft_amd_plan* p, void* Ax, void* Ay, void* Az, const void* A,
+                  unsigned mask, bool slow) {
+    void* outs[3] = {Ax, Ay, Az};
+    int st;
+    if (!slow) {
+      for (int d = 0; d < 3; ++d)
+        if ((mask & (1u << d)) && outs[d])
+          if ((st = deriv(p, d, A, outs[d], 0))) return st;
+      return ACCFFT_AMD_OK;
+    }
+    const int64_t N0 = p->n[0], N1 = p->n[1], N2 = p->n[2], Nh = N2 / 2 + 1;
+    const size_t cb = (size_t)N0 * N1 * Nh * sizeof(C);
+    if ((st = ensure_ws(p, 0, cb)) || (st = ensure_ws(p, 1, cb))) return st;
+    if ((st = z_r2c(p, A, p->ws[0], false))) return st;
+    if ((st = y_c2c(p, p->ws[0], p->ws[0], Nh, -1))) return st;
+    const double scale = 1.0 / ((double)N0 * N1 * N2);
+    for (int d = 0; d < 3; ++d) {
+      if (!(mask & (1u << d)) || !outs[d]) continue;
+      if ((st = x_mult(p, p->ws[0], p->ws[1], Nh, SYM_GRAD_X + d, scale))) return st;
+      if ((st = y_c2c(p, p->ws[1], p->ws[1], Nh, +1))) return st;
+      if ((st = z_c2r(p, p->ws[1], outs[d], false))) return st;
+    }
+    return ACCFFT_AMD_OK;
+  }
+
+  // 3-D r2c -> symbol -> 3-D c2r with the multiply fused into the x pass
+  static int spectral(accfft_amd_plan* p, int sym, void* out, const void* A) {
+    const int64_t N0 = p->n[0], N1 = p->n[1], N2 = p->n[2], Nh = N2 / 2 + 1;
+    int st;
+    if ((st = ensure_ws(p, 0, (size_t)N0 * N1 * Nh * sizeof(C)))) return st;
+    void* w = p->ws[0];
+    if ((st = z_r2c(p, A, w, false))) return st;
+    if ((st = y_c2c(p, w, w, Nh, -1))) return st;
+    if ((st = x_mult(p, w, w, Nh, sym, 1.0 / ((double)N0 * N1 * N2)))) return st;
+    if ((st = y_c2c(p, w, w, Nh, +1))) return st;
+    return z_c2r(p, w, out, false);
+  }
+
+  static int divergence(accfft_amd_plan* p, void* div, const void* Ax, const void* Ay,
+                        const void* Az, bool slow) {
+    int st;
+    if (!slow) {
+      if ((st = deriv(p, 0, Ax, div, 0))) return st;
+      if ((st = deriv(p, 1, Ay, div, 1))) return st;
+      return deriv(p, 2, Az, div, 1);
+    }
+    // operators_gpu.txx:336-446: component d transformed over axes {d, z}
+    const int64_t N0 = p->n[0], N1 = p->n[1], N2 = p->n[2], Nh = N2 / 2 + 1;
+    const int64_t R = N0 * N1 * N2;
+    if ((st = ensure_ws(p, 0, (size_t)N0 * N1 * Nh * sizeof(C))) ||
+        (st = ensure_ws(p, 1, (size_t)R * sizeof(T))))
+      return st;
+    void* w = p->ws[0];
+    if ((st = z_r2c(p, Ax, w, false))) return st;
+    if ((st = x_mult(p, w, w, Nh, SYM_GRAD_X, 1.0 / ((double)N0 * N2)))) return st;
+    if ((st = z_c2r(p, w, div, false))) return st;
+    if ((st = z_r2c(p, Ay, w, false))) return st;
+    if ((st = y_mult(p, w, w, Nh, SYM_GRAD_Y, 1.0 / ((double)N1 * N2)))) return st;
+    if ((st = z_c2r(p, w, p->ws[1], false))) return st;
+    if ((st = status_of(launch_axpy<T>(R, (T)1, (const T*)p->ws[1], (T*)div, p->stream))))
+      return st;
+    return deriv(p, 2, Az, div, 1);
+  }
+};
+
+template <>
+inline const AxisTab<float>& Ops<float>::tab(const accfft_amd_plan* p, int ax) {
+  return p->tf[ax];
+}
+template <>
+inline const AxisTab<double>& Ops<double>::tab(const accfft_amd_plan* p, int ax) {
+  return p->td[ax];
+}
+
+}  // namespace afft

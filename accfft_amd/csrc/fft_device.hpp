@@ -1,0 +1,463 @@
+// fft_device.hpp -- device-side building blocks of the gfx950 FFT kernels.
+//
+// The reference delegates all FFT arithmetic to cuFFT (cufftExecD2Z/Z2Z/Z2D
+// at src/accfft_gpu.cpp:480-593).  Here every 1-D transform is a Stockham
+// autosort DFT run inside LDS by one workgroup:
+//   stage with radix R and span NS (product of the earlier radices) reads
+//   element j + r*N/R, applies exp(sign*2*pi*i*r*(j%NS)/(NS*R)), does an
+//   R-point DFT and writes element (j/NS)*NS*R + j%NS + r*NS.
+// Two engines share that recurrence:
+//   * the static engine (power-of-two N, radix schedule fixed at compile
+//     time) keeps each thread's butterflies in VGPRs across the barrier and
+//     works in place in LDS, with the per-stage twiddles staged in LDS;
+//   * the dynamic engine (any N, schedule passed at run time) ping-pongs
+//     between two LDS buffers and reads twiddles from a length-N table.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace afft {
+
+template <typename T>
+struct alignas(2 * sizeof(T)) cpx {
+  T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> csub(cpx<T> a, cpx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ __forceinline__ cpx<T> cmul(cpx<T> a, cpx<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+// a * conj(b)
+template <typename T>
+__device__ __forceinline__ cpx<T> cmulc(cpx<T> a, cpx<T> b) {
+  return {a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+// multiply by SIGN*i
+template <int SIGN, typename T>
+__device__ __forceinline__ cpx<T> mul_si(cpx<T> a) {
+  if constexpr (SIGN > 0) return {-a.y, a.x};
+  else return {a.y, -a.x};
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> cscale(cpx<T> a, T s) { return {a.x * s, a.y * s}; }
+
+// ------------------------------------------------------------------
+// Small DFTs in registers, natural order in and out:
+//   X[q] = sum_r v[r] * exp(SIGN * 2*pi*i * r*q / R)
+// ------------------------------------------------------------------
+template <typename T, int SIGN>
+__device__ __forceinline__ void dft2(cpx<T>* v) {
+  cpx<T> a = v[0];
+  v[0] = cadd(a, v[1]);
+  v[1] = csub(a, v[1]);
+}
+
+template <typename T, int SIGN>
+__device__ __forceinline__ void dft4(cpx<T>* v) {
+  cpx<T> s0 = cadd(v[0], v[2]), d0 = csub(v[0], v[2]);
+  cpx<T> s1 = cadd(v[1], v[3]), d1 = mul_si<SIGN>(csub(v[1], v[3]));
+  v[0] = cadd(s0, s1);
+  v[2] = csub(s0, s1);
+  v[1] = cadd(d0, d1);
+  v[3] = csub(d0, d1);
+}
+
+// 8 = 2 x 4 (decimation in time)
+template <typename T, int SIGN>
+__device__ __forceinline__ void dft8(cpx<T>* v) {
+  const T c = (T)0.70710678118654752440;
+  cpx<T> e[4] = {v[0], v[2], v[4], v[6]};
+  cpx<T> o[4] = {v[1], v[3], v[5], v[7]};
+  dft4<T, SIGN>(e);
+  dft4<T, SIGN>(o);
+  // o[k] *= W8^k, W8 = exp(SIGN*i*pi/4)
+  cpx<T> o1 = {c * (o[1].x - SIGN * o[1].y), c * (o[1].y + SIGN * o[1].x)};
+  cpx<T> o2 = mul_si<SIGN>(o[2]);
+  cpx<T> o3 = {c * (-o[3].x - SIGN * o[3].y), c * (-o[3].y + SIGN * o[3].x)};
+  v[0] = cadd(e[0], o[0]);
+  v[4] = csub(e[0], o[0]);
+  v[1] = cadd(e[1], o1);
+  v[5] = csub(e[1], o1);
+  v[2] = cadd(e[2], o2);
+  v[6] = csub(e[2], o2);
+  v[3] = cadd(e[3], o3);
+  v[7] = csub(e[3], o3);
+}
+
+// 16 = 2 x 8 (decimation in time)
+template <typename T, int SIGN>
+__device__ __forceinline__ void dft16(cpx<T>* v) {
+  const T c1 = (T)0.92387953251128675613;  // cos(pi/8)
+  const T s1 = (T)0.38268343236508977173;  // sin(pi/8)
+  const T c2 = (T)0.70710678118654752440;
+  cpx<T> e[8], o[8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    e[r] = v[2 * r];
+    o[r] = v[2 * r + 1];
+  }
+  dft8<T, SIGN>(e);
+  dft8<T, SIGN>(o);
+  // W16^k = cos(pi k/8) + SIGN*i*sin(pi k/8)
+  const T cw[8] = {(T)1, c1, c2, s1, (T)0, -s1, -c2, -c1};
+  const T sw[8] = {(T)0, s1, c2, c1, (T)1, c1, c2, s1};
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    cpx<T> t;
+    if (k == 0) t = o[0];
+    else if (k == 4) t = mul_si<SIGN>(o[4]);
+    else {
+      const T wr = cw[k], wi = SIGN * sw[k];
+      t = {o[k].x * wr - o[k].y * wi, o[k].x * wi + o[k].y * wr};
+    }
+    v[k] = cadd(e[k], t);
+    v[k + 8] = csub(e[k], t);
+  }
+}
+
+// Odd prime R in {3, 5, 7} by the symmetric pair form:
+//   X[q]   = v0 + sum_r a_r cos(2 pi r q/R) + SIGN*i sum_r b_r sin(2 pi r q/R)
+//   X[R-q] = v0 + sum_r a_r cos(.)          - SIGN*i sum_r b_r sin(.)
+// with a_r = v[r] + v[R-r], b_r = v[r] - v[R-r].
+template <int R> struct OddConst;
+template <> struct OddConst<3> {
+  static __device__ __forceinline__ double c(int m) { return -0.5; }
+  static __device__ __forceinline__ double s(int m) {
+    return m == 1 ? 0.86602540378443864676 : -0.86602540378443864676;
+  }
+};
+template <> struct OddConst<5> {
+  static __device__ __forceinline__ double c(int m) {
+    return (m == 1 || m == 4) ? 0.30901699437494742410 : -0.80901699437494742410;
+  }
+  static __device__ __forceinline__ double s(int m) {
+    return m == 1 ? 0.95105651629515357212
+         : m == 2 ? 0.58778525229247312917
+         : m == 3 ? -0.58778525229247312917
+                  : -0.95105651629515357212;
+  }
+};
+template <> struct OddConst<7> {
+  static __device__ __forceinline__ double c(int m) {
+    return (m == 1 || m == 6) ? 0.62348980185873353053
+         : (m == 2 || m == 5) ? -0.22252093395631440429
+                              : -0.90096886790241912624;
+  }
+  static __device__ __forceinline__ double s(int m) {
+    const double t[7] = {0.0, 0.78183148246802980871, 0.97492791218182360702,
+                         0.43388373911755812048, -0.43388373911755812048,
+                         -0.97492791218182360702, -0.78183148246802980871};
+    return t[m];
+  }
+};
+
+template <typename T, int R, int SIGN>
+__device__ __forceinline__ void dft_odd(cpx<T>* v) {
+  constexpr int H = (R - 1) / 2;
+  cpx<T> a[H], b[H];
+  cpx<T> v0 = v[0];
+  cpx<T> sum = v[0];
+#pragma unroll
+  for (int r = 1; r <= H; ++r) {
+    a[r - 1] = cadd(v[r], v[R - r]);
+    b[r - 1] = csub(v[r], v[R - r]);
+    sum = cadd(sum, a[r - 1]);
+  }
+  v[0] = sum;
+#pragma unroll
+  for (int q = 1; q <= H; ++q) {
+    cpx<T> ca = v0, sb = {(T)0, (T)0};
+#pragma unroll
+    for (int r = 1; r <= H; ++r) {
+      const int m = (r * q) % R;
+      const T cc = (T)OddConst<R>::c(m), ss = (T)OddConst<R>::s(m);
+      ca.x += a[r - 1].x * cc;
+      ca.y += a[r - 1].y * cc;
+      sb.x += b[r - 1].x * ss;
+      sb.y += b[r - 1].y * ss;
+    }
+    cpx<T> isb = mul_si<SIGN>(sb);
+    v[q] = cadd(ca, isb);
+    v[R - q] = csub(ca, isb);
+  }
+}
+
+template <typename T, int R, int SIGN>
+__device__ __forceinline__ void dft(cpx<T>* v) {
+  if constexpr (R == 2) dft2<T, SIGN>(v);
+  else if constexpr (R == 4) dft4<T, SIGN>(v);
+  else if constexpr (R == 8) dft8<T, SIGN>(v);
+  else if constexpr (R == 16) dft16<T, SIGN>(v);
+  else dft_odd<T, R, SIGN>(v);
+}
+
+// ------------------------------------------------------------------
+// Static engine: compile-time radix schedules for power-of-two N.
+// ------------------------------------------------------------------
+template <int N> struct Sched;
+template <> struct Sched<16> { static constexpr int S = 2; static constexpr int R[3] = {4, 4, 0}; };
+template <> struct Sched<32> { static constexpr int S = 2; static constexpr int R[3] = {8, 4, 0}; };
+template <> struct Sched<64> { static constexpr int S = 2; static constexpr int R[3] = {8, 8, 0}; };
+template <> struct Sched<128> { static constexpr int S = 2; static constexpr int R[3] = {8, 16, 0}; };
+template <> struct Sched<256> { static constexpr int S = 2; static constexpr int R[3] = {16, 16, 0}; };
+template <> struct Sched<512> { static constexpr int S = 3; static constexpr int R[3] = {8, 8, 8}; };
+template <> struct Sched<1024> { static constexpr int S = 3; static constexpr int R[3] = {16, 8, 8}; };
+template <> struct Sched<2048> { static constexpr int S = 3; static constexpr int R[3] = {16, 16, 8}; };
+template <> struct Sched<4096> { static constexpr int S = 3; static constexpr int R[3] = {16, 16, 16}; };
+
+template <int N>
+__host__ __device__ constexpr int sched_ns(int s) {  // span before stage s
+  int ns = 1;
+  for (int i = 0; i < s; ++i) ns *= Sched<N>::R[i];
+  return ns;
+}
+template <int N>
+__host__ __device__ constexpr int sched_twoff(int s) {  // twiddle table offset of stage s
+  int off = 0;
+  for (int i = 1; i < s; ++i) off += (Sched<N>::R[i] - 1) * sched_ns<N>(i);
+  return off;
+}
+template <int N>
+__host__ __device__ constexpr int sched_twn() { return sched_twoff<N>(Sched<N>::S); }
+template <int N>
+__host__ __device__ constexpr int sched_minnb() {  // fewest butterflies in any stage
+  int m = N;
+  for (int i = 0; i < Sched<N>::S; ++i)
+    if (N / Sched<N>::R[i] < m) m = N / Sched<N>::R[i];
+  return m;
+}
+
+// Accessor over one line in LDS with compile-time element stride ES.
+template <typename T, int ES>
+struct LineAcc {
+  cpx<T>* p;
+  __device__ __forceinline__ cpx<T> ld(int n) const { return p[n * ES]; }
+  __device__ __forceinline__ void st(int n, cpx<T> v) const { p[n * ES] = v; }
+};
+
+// One in-place stage.  Thread q of NT threads on the line performs
+// butterflies j = q + NT*i.  Twiddles for stage s live at tw[TWOFF +
+// (r-1)*NS + k] = exp(-2 pi i r k / (NS R)); conjugated for SIGN = +1.
+template <typename T, int N, int R, int NS, int NT, int SIGN, int TWOFF, int ES>
+__device__ __forceinline__ void sstage(const LineAcc<T, ES>& acc, int q,
+                                       const cpx<T>* __restrict__ tw) {
+  constexpr int NB = N / R;
+  constexpr int PER = NB / NT;
+  static_assert(NB % NT == 0, "threads per line must divide the butterfly count");
+  cpx<T> v[PER][R];
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int j = q + NT * i;
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[i][r] = acc.ld(j + r * NB);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const int j = q + NT * i;
+    const int k = (NS == 1) ? 0 : (j & (NS - 1));
+    if constexpr (NS > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) {
+        const cpx<T> w = tw[TWOFF + (r - 1) * NS + k];
+        v[i][r] = (SIGN < 0) ? cmul(v[i][r], w) : cmulc(v[i][r], w);
+      }
+    }
+    dft<T, R, SIGN>(v[i]);
+    const int base = (j - k) * R + k;  // (j/NS)*NS*R + k
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc.st(base + r * NS, v[i][r]);
+  }
+  __syncthreads();
+}
+
+// Full transform of one line held in LDS (natural order in and out).
+// Caller must __syncthreads() before (data and twiddles staged).
+template <typename T, int N, int NT, int SIGN, int ES>
+__device__ __forceinline__ void static_fft(const LineAcc<T, ES>& acc, int q,
+                                           const cpx<T>* __restrict__ tw) {
+  using S = Sched<N>;
+  sstage<T, N, S::R[0], 1, NT, SIGN, 0, ES>(acc, q, tw);
+  if constexpr (S::S > 1)
+    sstage<T, N, S::R[1], sched_ns<N>(1), NT, SIGN, sched_twoff<N>(1), ES>(acc, q, tw);
+  if constexpr (S::S > 2)
+    sstage<T, N, S::R[2], sched_ns<N>(2), NT, SIGN, sched_twoff<N>(2), ES>(acc, q, tw);
+}
+
+// ------------------------------------------------------------------
+// Dynamic engine: any N up to the LDS budget; schedule at run time.
+// ------------------------------------------------------------------
+struct DynSched {
+  int n;
+  int nst;
+  int rad[20];
+};
+
+// Flat butterfly index b -> (line, j).  Layout Z: line stride LS, element
+// stride 1, threads walk j fastest.  Layout S: element stride W, line (= the
+// column inside the tile) fastest.
+struct LayZ {
+  int ls;
+  __device__ __forceinline__ int addr(int line, int n) const { return line * ls + n; }
+  __device__ __forceinline__ void map(int b, int nb, int& line, int& j) const {
+    line = b / nb;
+    j = b - line * nb;
+  }
+};
+struct LayS {
+  int w;
+  __device__ __forceinline__ int addr(int line, int n) const { return n * w + line; }
+  __device__ __forceinline__ void map(int b, int nb, int& line, int& j) const {
+    j = b / w;
+    line = b - j * w;
+  }
+};
+
+template <typename T, int R, int SIGN, class Lay>
+__device__ __forceinline__ void dstage(const cpx<T>* src, cpx<T>* dst, int N, int NS,
+                                       int nlines, const Lay& lay,
+                                       const cpx<T>* __restrict__ twg) {
+  const int NB = N / R;
+  const int total = nlines * NB;
+  const int tstep = N / (NS * R);
+  for (int b = threadIdx.x; b < total; b += blockDim.x) {
+    int line, j;
+    lay.map(b, NB, line, j);
+    const int k = j % NS;
+    cpx<T> v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      cpx<T> x = src[lay.addr(line, j + r * NB)];
+      if (r > 0 && NS > 1) {
+        const cpx<T> w = twg[(r * k * tstep) % N];
+        x = (SIGN < 0) ? cmul(x, w) : cmulc(x, w);
+      }
+      v[r] = x;
+    }
+    dft<T, R, SIGN>(v);
+    const int base = (j - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) dst[lay.addr(line, base + r * NS)] = v[r];
+  }
+}
+
+// Generic radix (any R, O(R^2) per butterfly) straight from LDS.
+template <typename T, int SIGN, class Lay>
+__device__ __forceinline__ void dstage_generic(const cpx<T>* src, cpx<T>* dst, int N, int R,
+                                               int NS, int nlines, const Lay& lay,
+                                               const cpx<T>* __restrict__ twg) {
+  const int NB = N / R;
+  const int total = nlines * NB;
+  const int tstep = N / (NS * R);
+  const int rstep = N / R;
+  for (int b = threadIdx.x; b < total; b += blockDim.x) {
+    int line, j;
+    lay.map(b, NB, line, j);
+    const int k = j % NS;
+    const int base = (j - k) * R + k;
+    for (int q = 0; q < R; ++q) {
+      cpx<T> acc = {(T)0, (T)0};
+      for (int r = 0; r < R; ++r) {
+        const long m = ((long)r * k * tstep + (long)((r * q) % R) * rstep) % N;
+        const cpx<T> w = twg[m];
+        const cpx<T> x = src[lay.addr(line, j + r * NB)];
+        const cpx<T> t = (SIGN < 0) ? cmul(x, w) : cmulc(x, w);
+        acc = cadd(acc, t);
+      }
+      dst[lay.addr(line, base + q * NS)] = acc;
+    }
+  }
+}
+
+// Runs the whole transform; data starts in a, the returned pointer holds the
+// result (a or b).  Caller must __syncthreads() before; returns after a
+// barrier.
+template <typename T, int SIGN, class Lay>
+__device__ cpx<T>* dyn_fft(cpx<T>* a, cpx<T>* b, const DynSched& sc, int nlines,
+                           const Lay& lay, const cpx<T>* __restrict__ twg) {
+  int NS = 1;
+  const int N = sc.n;
+  for (int s = 0; s < sc.nst; ++s) {
+    const int R = sc.rad[s];
+    switch (R) {
+      case 2: dstage<T, 2, SIGN>(a, b, N, NS, nlines, lay, twg); break;
+      case 3: dstage<T, 3, SIGN>(a, b, N, NS, nlines, lay, twg); break;
+      case 4: dstage<T, 4, SIGN>(a, b, N, NS, nlines, lay, twg); break;
+      case 5: dstage<T, 5, SIGN>(a, b, N, NS, nlines, lay, twg); break;
+      case 7: dstage<T, 7, SIGN>(a, b, N, NS, nlines, lay, twg); break;
+      case 8: dstage<T, 8, SIGN>(a, b, N, NS, nlines, lay, twg); break;
+      case 16: dstage<T, 16, SIGN>(a, b, N, NS, nlines, lay, twg); break;
+      default: dstage_generic<T, SIGN>(a, b, N, R, NS, nlines, lay, twg); break;
+    }
+    __syncthreads();
+    cpx<T>* t = a;
+    a = b;
+    b = t;
+    NS *= R;
+  }
+  return a;
+}
+
+// ------------------------------------------------------------------
+// Spectral symbols (src/operators_cuda.cu:15-205, operators.txx:276-392)
+// ------------------------------------------------------------------
+enum SymKind {
+  SYM_GRAD_X = 0,
+  SYM_GRAD_Y = 1,
+  SYM_GRAD_Z = 2,
+  SYM_LAPLACE = 3,
+  SYM_BIHARMONIC = 4,
+  SYM_INV_LAPLACE = 5,
+  SYM_INV_BIHARMONIC = 6
+};
+
+// w = X; if X > N/2: w -= N; if X == N/2: w = 0  (operators_cuda.cu:32-37)
+__device__ __forceinline__ long wave_nyq0(long X, long N) {
+  long w = X;
+  if (X > N / 2) w -= N;
+  if (X == N / 2) w = 0;
+  return w;
+}
+// Odd-symmetric wavenumber of the per-axis r2c -> i*w -> c2r chain
+// (operators_gpu.txx:172-278): modes k and N-k are both zeroed when either
+// equals N/2 (integer division).
+__device__ __forceinline__ long wave_sym(long k, long N) {
+  if (k == N / 2 || N - k == N / 2) return 0;
+  return k > N / 2 ? k - N : k;
+}
+
+// Applies symbol `kind` at global (X, Y, Z) with FP64 factor arithmetic as
+// the reference, result stored in T.
+template <typename T>
+__device__ __forceinline__ cpx<T> apply_sym(int kind, long X, long Y, long Z, long N0, long N1,
+                                            long N2, double scale, cpx<T> v) {
+  if (kind <= SYM_GRAD_Z) {
+    const long w = kind == SYM_GRAD_X ? wave_nyq0(X, N0)
+                 : kind == SYM_GRAD_Y ? wave_nyq0(Y, N1)
+                                      : wave_nyq0(Z, N2);
+    const T f = (T)(scale * (double)w);
+    return {-f * v.y, f * v.x};
+  }
+  double f;
+  if (kind == SYM_INV_BIHARMONIC) {
+    long wx = X > N0 / 2 ? X - N0 : X, wy = Y > N1 / 2 ? Y - N1 : Y, wz = Z > N2 / 2 ? Z - N2 : Z;
+    double wave = (double)(-wx * wx - wy * wy - wz * wz);
+    wave *= wave;
+    if (wave == 0) wave = 1;
+    f = 1.0 / wave;
+  } else {
+    const long wx = wave_nyq0(X, N0), wy = wave_nyq0(Y, N1), wz = wave_nyq0(Z, N2);
+    const long wave = -wx * wx - wy * wy - wz * wz;
+    if (kind == SYM_LAPLACE) f = (double)wave;
+    else if (kind == SYM_BIHARMONIC) f = (double)(wave * wave);
+    else f = wave == 0 ? 0.0 : 1.0 / (double)wave;
+  }
+  const T g = (T)(scale * f);
+  return {g * v.x, g * v.y};
+}
+
+}  // namespace afft

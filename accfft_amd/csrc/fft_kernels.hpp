@@ -1,0 +1,407 @@
+// fft_kernels.hpp -- batched 1-D FFT kernels for the three axis passes.
+//
+// zfft_*  : lines contiguous in memory (the z pass; cuFFT fplan_0/iplan_0 of
+//           src/accfft_gpu.cpp:173-206 and the c2c fplan_0 of :767-789).
+// sfft_*  : lines strided in memory (the y and x passes; fplan_1/fplan_2 of
+//           :208-253, the per-axis fplan_y/fplan_x of :255-323).  A tile is
+//           W consecutive columns x N rows, so every row segment a wavefront
+//           touches is W contiguous elements.
+//
+// Modes:
+//   M_C2C   complex in -> DFT(SIGN) -> complex out
+//   M_R2C   two real lines a,b packed as a+ib -> DFT(-1) -> the two half
+//           spectra A[k] = (Z[k]+conj Z[N-k])/2, B[k] = -i(Z[k]-conj Z[N-k])/2
+//   M_C2R   two half spectra -> Z = A + iB (Hermitian extension, Im of DC
+//           and Nyquist ignored) -> DFT(+1) -> Re to line a, Im to line b
+//   M_MULT  complex in -> DFT(-1) -> spectral symbol -> DFT(+1) -> out
+//           (the multiply of operators_gpu.txx fused between the forward
+//           and inverse x passes)
+//   M_DERIV two real lines -> DFT(-1) -> i*w_sym(k)*scale -> DFT(+1) -> real
+//           (the per-axis r2c / grad_mult / c2r chain of
+//           operators_gpu.txx:172-278 in one pass, optional accumulate)
+#pragma once
+#include "fft_device.hpp"
+
+namespace afft {
+
+enum Mode { M_C2C = 0, M_R2C = 1, M_C2R = 2, M_MULT = 3, M_DERIV = 4 };
+
+template <typename T>
+struct FftArgs {
+  const void* in;
+  void* out;
+  const cpx<T>* tw;  // static engine: per-stage table; dynamic: exp(-2 pi i m/n)
+  DynSched sc;       // sc.n is the transform length for both engines
+  int64_t nlines;    // z: lines (real lines for pair modes); s: columns G
+  int64_t idist, odist;              // z: line distance (elements of in/out type)
+  int64_t C, Din, Dout, Sin, Sout;   // s: column g -> (g/C)*D + g%C, row stride S
+  int lpw;                           // dynamic: lines (z) or columns (s) per block
+  int sym;                           // M_MULT symbol kind
+  double scale;                      // M_MULT / M_DERIV scale
+  long N0, N1, N2;                   // M_MULT global sizes
+  int64_t Ch;                        // M_MULT: column g -> (g/Ch, g%Ch)
+  int maxis;                         // M_MULT: axis of the rows (0: x, 1: y)
+  int accumulate;                    // M_DERIV: out += result
+};
+
+// -------------------------------------------------------------- configs
+template <int N>
+struct ZCfg {
+  static constexpr int NT = sched_minnb<N>();
+  static constexpr int TPB = NT > 256 ? NT : 256;
+  static constexpr int LPW = TPB / NT;
+};
+
+template <typename T, int N>
+struct SCfg {
+  static constexpr int TPB = 256;
+  // keep the tile at <= 64 KiB of LDS and at least one row per thread group
+  static constexpr int W0 = (int)(65536 / (N * (int)sizeof(cpx<T>)));
+  static constexpr int WMAX = W0 > 64 ? 64 : W0;
+  static constexpr int WMIN = TPB / sched_minnb<N>();
+  static constexpr int W = WMAX > WMIN ? WMAX : WMIN;
+  static constexpr int NT = TPB / W;
+};
+
+template <typename T>
+__device__ __forceinline__ cpx<T> ldc(const void* p, int64_t i) {
+  return reinterpret_cast<const cpx<T>*>(p)[i];
+}
+template <typename T>
+__device__ __forceinline__ T ldr(const void* p, int64_t i) {
+  return reinterpret_cast<const T*>(p)[i];
+}
+template <typename T>
+__device__ __forceinline__ void stc(void* p, int64_t i, cpx<T> v) {
+  reinterpret_cast<cpx<T>*>(p)[i] = v;
+}
+template <typename T>
+__device__ __forceinline__ void str(void* p, int64_t i, T v) {
+  reinterpret_cast<T*>(p)[i] = v;
+}
+
+// -------------------------------------------------------------- engines
+// N > 0: static engine; N == 0: dynamic engine.
+template <typename T, int N, int NT, int SIGN>
+__device__ __forceinline__ cpx<T>* zrun(cpx<T>* A, cpx<T>* B, const FftArgs<T>& a,
+                                        const cpx<T>* twl, int lpw) {
+  if constexpr (N > 0) {
+    const int line = threadIdx.x / NT, q = threadIdx.x % NT;
+    LineAcc<T, 1> acc{A + line * N};
+    static_fft<T, N, NT, SIGN, 1>(acc, q, twl);
+    return A;
+  } else {
+    LayZ lay{a.sc.n};
+    return dyn_fft<T, SIGN>(A, B, a.sc, lpw, lay, a.tw);
+  }
+}
+
+template <typename T, int N, int W, int SIGN>
+__device__ __forceinline__ cpx<T>* srun(cpx<T>* A, cpx<T>* B, const FftArgs<T>& a,
+                                        const cpx<T>* twl, int w_dyn) {
+  if constexpr (N > 0) {
+    const int w = threadIdx.x % W, q = threadIdx.x / W;
+    LineAcc<T, W> acc{A + w};
+    static_fft<T, N, 256 / W, SIGN, W>(acc, q, twl);
+    return A;
+  } else {
+    LayS lay{w_dyn};
+    return dyn_fft<T, SIGN>(A, B, a.sc, w_dyn, lay, a.tw);
+  }
+}
+
+// ====================================================================
+// Contiguous lines (z)
+// ====================================================================
+template <typename T, int N, int MODE, int SIGN>
+__global__ void __launch_bounds__(N > 0 ? ZCfg<(N > 0 ? N : 16)>::TPB : 256)
+    zfft_kernel(FftArgs<T> a) {
+  constexpr bool STATIC = N > 0;
+  constexpr int NS_ = STATIC ? N : 16;
+  constexpr int NT = ZCfg<NS_>::NT;
+  constexpr int TPB = STATIC ? ZCfg<NS_>::TPB : 256;
+  constexpr int TWN = STATIC ? sched_twn<NS_>() : 0;
+  constexpr bool PAIR = (MODE == M_R2C || MODE == M_C2R || MODE == M_DERIV);
+  const int n = STATIC ? N : a.sc.n;
+  const int nh = n / 2 + 1;
+  const int lpw = STATIC ? ZCfg<NS_>::LPW : a.lpw;  // complex lines per block
+  const int t = threadIdx.x;
+
+  cpx<T>* bufA;
+  cpx<T>* bufB;
+  const cpx<T>* twl;
+  if constexpr (STATIC) {
+    constexpr int LPW = ZCfg<NS_>::LPW;
+    __shared__ cpx<T> s_buf[LPW * NS_];
+    __shared__ cpx<T> s_tw[TWN > 0 ? TWN : 1];
+    for (int i = t; i < TWN; i += TPB) s_tw[i] = a.tw[i];
+    bufA = s_buf;
+    bufB = s_buf;
+    twl = s_tw;
+  } else {
+    extern __shared__ float4 dsm[];
+    bufA = reinterpret_cast<cpx<T>*>(dsm);
+    bufB = bufA + lpw * n;
+    twl = a.tw;
+  }
+
+  const int64_t l0 = (int64_t)blockIdx.x * lpw;  // first complex line / pair
+  const int64_t nl = a.nlines;
+
+  // ---------------- load
+  if constexpr (MODE == M_C2C || MODE == M_MULT) {
+    for (int e = t; e < lpw * n; e += TPB) {
+      const int p = e / n, i = e - p * n;
+      const int64_t g = l0 + p;
+      cpx<T> v = {(T)0, (T)0};
+      if (g < nl) v = ldc<T>(a.in, g * a.idist + i);
+      bufA[e] = v;
+    }
+  } else if constexpr (MODE == M_R2C || MODE == M_DERIV) {
+    for (int e = t; e < lpw * n; e += TPB) {
+      const int p = e / n, i = e - p * n;
+      const int64_t ra = 2 * (l0 + p), rb = ra + 1;
+      cpx<T> v = {(T)0, (T)0};
+      if (ra < nl) v.x = ldr<T>(a.in, ra * a.idist + i);
+      if (rb < nl) v.y = ldr<T>(a.in, rb * a.idist + i);
+      bufA[e] = v;
+    }
+  } else {  // M_C2R: Z = A + iB from the two half spectra (Hermitian extension)
+    for (int e = t; e < lpw * n; e += TPB) {
+      const int p = e / n, i = e - p * n;
+      const int64_t ga = 2 * (l0 + p), gb = ga + 1;
+      const int k = i < nh ? i : n - i;
+      cpx<T> A = {(T)0, (T)0}, B = {(T)0, (T)0};
+      if (ga < nl) A = ldc<T>(a.in, ga * a.idist + k);
+      if (gb < nl) B = ldc<T>(a.in, gb * a.idist + k);
+      if (k == 0 || 2 * k == n) { A.y = (T)0; B.y = (T)0; }
+      bufA[e] = i < nh ? cpx<T>{A.x - B.y, A.y + B.x} : cpx<T>{A.x + B.y, B.x - A.y};
+    }
+  }
+  __syncthreads();
+
+  // ---------------- transform
+  cpx<T>* res;
+  if constexpr (MODE == M_C2C) {
+    res = zrun<T, N, NT, SIGN>(bufA, bufB, a, twl, lpw);
+  } else if constexpr (MODE == M_R2C) {
+    res = zrun<T, N, NT, -1>(bufA, bufB, a, twl, lpw);
+  } else if constexpr (MODE == M_C2R) {
+    res = zrun<T, N, NT, +1>(bufA, bufB, a, twl, lpw);
+  } else {  // M_DERIV and M_MULT (z has no MULT use; treat as DERIV-style symbol)
+    res = zrun<T, N, NT, -1>(bufA, bufB, a, twl, lpw);
+    for (int e = t; e < lpw * n; e += TPB) {
+      const int p = e / n, i = e - p * n;
+      (void)p;
+      const T f = (T)(a.scale * (double)wave_sym(i, n));
+      const cpx<T> v = res[e];
+      res[e] = {-f * v.y, f * v.x};
+    }
+    __syncthreads();
+    cpx<T>* other = (res == bufA) ? bufB : bufA;
+    res = zrun<T, N, NT, +1>(res, other, a, twl, lpw);
+  }
+
+  // ---------------- store
+  if constexpr (MODE == M_C2C || MODE == M_MULT) {
+    for (int e = t; e < lpw * n; e += TPB) {
+      const int p = e / n, i = e - p * n;
+      const int64_t g = l0 + p;
+      if (g < nl) stc<T>(a.out, g * a.odist + i, res[e]);
+    }
+  } else if constexpr (MODE == M_R2C) {
+    for (int e = t; e < 2 * lpw * nh; e += TPB) {
+      const int rl = e / nh, k = e - rl * nh;
+      const int p = rl >> 1;
+      const int64_t g = 2 * l0 + rl;
+      if (g >= nl) continue;
+      const cpx<T> z1 = res[p * n + k];
+      const cpx<T> z2 = res[p * n + (k == 0 ? 0 : n - k)];
+      const T h = (T)0.5;
+      cpx<T> o;
+      if ((rl & 1) == 0) o = {h * (z1.x + z2.x), h * (z1.y - z2.y)};
+      else o = {h * (z1.y + z2.y), h * (z2.x - z1.x)};
+      stc<T>(a.out, g * a.odist + k, o);
+    }
+  } else {  // real outputs
+    for (int e = t; e < lpw * n; e += TPB) {
+      const int p = e / n, i = e - p * n;
+      const int64_t ra = 2 * (l0 + p), rb = ra + 1;
+      const cpx<T> z = res[e];
+      if constexpr (MODE == M_DERIV) {
+        if (a.accumulate) {
+          if (ra < nl) str<T>(a.out, ra * a.odist + i, ldr<T>(a.out, ra * a.odist + i) + z.x);
+          if (rb < nl) str<T>(a.out, rb * a.odist + i, ldr<T>(a.out, rb * a.odist + i) + z.y);
+          continue;
+        }
+      }
+      if (ra < nl) str<T>(a.out, ra * a.odist + i, z.x);
+      if (rb < nl) str<T>(a.out, rb * a.odist + i, z.y);
+    }
+  }
+}
+
+// ====================================================================
+// Strided lines (y, x)
+// ====================================================================
+template <typename T, int N, int MODE, int SIGN>
+__global__ void __launch_bounds__(256) sfft_kernel(FftArgs<T> a) {
+  constexpr bool STATIC = N > 0;
+  constexpr int NS_ = STATIC ? N : 16;
+  constexpr int TPB = 256;
+  constexpr int WS = SCfg<T, NS_>::W;
+  constexpr int TWN = STATIC ? sched_twn<NS_>() : 0;
+  const int n = STATIC ? N : a.sc.n;
+  const int nh = n / 2 + 1;
+  const int W = STATIC ? WS : a.lpw;  // complex columns per tile
+  const int t = threadIdx.x;
+
+  cpx<T>* bufA;
+  cpx<T>* bufB;
+  const cpx<T>* twl;
+  if constexpr (STATIC) {
+    __shared__ cpx<T> s_buf[WS * NS_];
+    __shared__ cpx<T> s_tw[TWN > 0 ? TWN : 1];
+    for (int i = t; i < TWN; i += TPB) s_tw[i] = a.tw[i];
+    bufA = s_buf;
+    bufB = s_buf;
+    twl = s_tw;
+  } else {
+    extern __shared__ float4 dsm[];
+    bufA = reinterpret_cast<cpx<T>*>(dsm);
+    bufB = bufA + W * n;
+    twl = a.tw;
+  }
+
+  // Each thread keeps one column w for the whole block (TPB % W == 0).
+  const int w = t % W;
+  const int r0 = t / W, rstep = TPB / W;
+  const int64_t G = a.nlines;
+  constexpr bool PAIR = (MODE == M_R2C || MODE == M_C2R || MODE == M_DERIV);
+  int64_t ga, gb;  // columns (real columns for pair modes)
+  if constexpr (PAIR) {
+    ga = 2 * ((int64_t)blockIdx.x * W + w);
+    gb = ga + 1;
+  } else {
+    ga = (int64_t)blockIdx.x * W + w;
+    gb = ga;
+  }
+  const bool va = ga < G, vb = gb < G;
+  const int64_t ia = (ga / a.C) * a.Din + (ga % a.C), ib = (gb / a.C) * a.Din + (gb % a.C);
+  const int64_t oa = (ga / a.C) * a.Dout + (ga % a.C), ob = (gb / a.C) * a.Dout + (gb % a.C);
+
+  // ---------------- load
+  if constexpr (MODE == M_C2C || MODE == M_MULT) {
+    for (int r = r0; r < n; r += rstep) {
+      cpx<T> v = {(T)0, (T)0};
+      if (va) v = ldc<T>(a.in, ia + (int64_t)r * a.Sin);
+      bufA[r * W + w] = v;
+    }
+  } else if constexpr (MODE == M_R2C || MODE == M_DERIV) {
+    for (int r = r0; r < n; r += rstep) {
+      cpx<T> v = {(T)0, (T)0};
+      if (va) v.x = ldr<T>(a.in, ia + (int64_t)r * a.Sin);
+      if (vb) v.y = ldr<T>(a.in, ib + (int64_t)r * a.Sin);
+      bufA[r * W + w] = v;
+    }
+  } else {  // M_C2R
+    for (int r = r0; r < n; r += rstep) {
+      const int k = r < nh ? r : n - r;
+      cpx<T> A = {(T)0, (T)0}, B = {(T)0, (T)0};
+      if (va) A = ldc<T>(a.in, ia + (int64_t)k * a.Sin);
+      if (vb) B = ldc<T>(a.in, ib + (int64_t)k * a.Sin);
+      if (k == 0 || 2 * k == n) { A.y = (T)0; B.y = (T)0; }
+      cpx<T> z;
+      if (r < nh) z = {A.x - B.y, A.y + B.x};
+      else z = {A.x + B.y, B.x - A.y};
+      bufA[r * W + w] = z;
+    }
+  }
+  __syncthreads();
+
+  // ---------------- transform
+  cpx<T>* res;
+  if constexpr (MODE == M_C2C) {
+    res = srun<T, N, WS, SIGN>(bufA, bufB, a, twl, W);
+  } else if constexpr (MODE == M_R2C) {
+    res = srun<T, N, WS, -1>(bufA, bufB, a, twl, W);
+  } else if constexpr (MODE == M_C2R) {
+    res = srun<T, N, WS, +1>(bufA, bufB, a, twl, W);
+  } else {
+    res = srun<T, N, WS, -1>(bufA, bufB, a, twl, W);
+    for (int r = r0; r < n; r += rstep) {
+      const cpx<T> v = res[r * W + w];
+      cpx<T> o;
+      if constexpr (MODE == M_DERIV) {
+        const T f = (T)(a.scale * (double)wave_sym(r, n));
+        o = {-f * v.y, f * v.x};
+      } else {
+        const long u = (long)(ga / a.Ch), Z = (long)(ga % a.Ch);
+        const long X = a.maxis == 0 ? (long)r : u, Y = a.maxis == 0 ? u : (long)r;
+        o = apply_sym<T>(a.sym, X, Y, Z, a.N0, a.N1, a.N2, a.scale, v);
+      }
+      res[r * W + w] = o;
+    }
+    __syncthreads();
+    cpx<T>* other = (res == bufA) ? bufB : bufA;
+    res = srun<T, N, WS, +1>(res, other, a, twl, W);
+  }
+
+  // ---------------- store
+  if constexpr (MODE == M_C2C || MODE == M_MULT) {
+    if (va)
+      for (int r = r0; r < n; r += rstep) stc<T>(a.out, oa + (int64_t)r * a.Sout, res[r * W + w]);
+  } else if constexpr (MODE == M_R2C) {
+    const T h = (T)0.5;
+    for (int k = r0; k < nh; k += rstep) {
+      const cpx<T> z1 = res[k * W + w];
+      const cpx<T> z2 = res[(k == 0 ? 0 : n - k) * W + w];
+      if (va) stc<T>(a.out, oa + (int64_t)k * a.Sout, cpx<T>{h * (z1.x + z2.x), h * (z1.y - z2.y)});
+      if (vb) stc<T>(a.out, ob + (int64_t)k * a.Sout, cpx<T>{h * (z1.y + z2.y), h * (z2.x - z1.x)});
+    }
+  } else {
+    for (int r = r0; r < n; r += rstep) {
+      const cpx<T> z = res[r * W + w];
+      if constexpr (MODE == M_DERIV) {
+        if (a.accumulate) {
+          if (va) str<T>(a.out, oa + (int64_t)r * a.Sout, ldr<T>(a.out, oa + (int64_t)r * a.Sout) + z.x);
+          if (vb) str<T>(a.out, ob + (int64_t)r * a.Sout, ldr<T>(a.out, ob + (int64_t)r * a.Sout) + z.y);
+          continue;
+        }
+      }
+      if (va) str<T>(a.out, oa + (int64_t)r * a.Sout, z.x);
+      if (vb) str<T>(a.out, ob + (int64_t)r * a.Sout, z.y);
+    }
+  }
+}
+
+// ====================================================================
+// Element-wise kernels
+// ====================================================================
+// out = symbol(kind) * scale * in over a complex block [osize] at global
+// offset ostart (src/operators_cuda.cu:15-205; operators.txx:276-392).
+template <typename T>
+__global__ void __launch_bounds__(256)
+    mult_kernel(const cpx<T>* __restrict__ in, cpx<T>* __restrict__ out, int kind, long N0,
+                long N1, long N2, long os0, long os1, long os2, long st0, long st1, long st2,
+                double scale) {
+  const int64_t total = (int64_t)os0 * os1 * os2;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < total;
+       p += (int64_t)gridDim.x * blockDim.x) {
+    const long k = (long)(p % os2);
+    const long j = (long)((p / os2) % os1);
+    const long i = (long)(p / ((int64_t)os1 * os2));
+    out[p] = apply_sym<T>(kind, i + st0, j + st1, k + st2, N0, N1, N2, scale, in[p]);
+  }
+}
+
+// y += alpha * x (daxpy_cu, src/operators_cuda.cu:207-215)
+template <typename T>
+__global__ void __launch_bounds__(256)
+    axpy_kernel(int64_t n, T alpha, const T* __restrict__ x, T* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    y[i] += alpha * x[i];
+}
+
+}  // namespace afft

@@ -1,0 +1,165 @@
+// fft_launch.hpp -- host-side selection of an FFT kernel instance and its
+// launch shape.  Included once per precision (fft_f32.hip, fft_f64.hip).
+#pragma once
+#include "fft_kernels.hpp"
+
+namespace afft {
+
+// Twiddle tables for one transform length, resident in device memory.
+template <typename T>
+struct AxisTab {
+  int n = 0;
+  bool is_static = false;
+  DynSched sc{};
+  const cpx<T>* tw_static = nullptr;  // per-stage table (static engine)
+  const cpx<T>* tw_full = nullptr;    // exp(-2 pi i m/n), m < n
+};
+
+inline bool static_length(int n) {
+  return n >= 16 && n <= 4096 && (n & (n - 1)) == 0;
+}
+
+inline bool pair_mode(int mode) { return mode == M_R2C || mode == M_C2R || mode == M_DERIV; }
+
+inline int dyn_lines(int n) {
+  int l = 4096 / n;
+  if (l < 1) l = 1;
+  if (l > 64) l = 64;
+  return l;
+}
+inline int dyn_cols(int n) {
+  int w = 1;
+  while (w * 2 <= 64 && w * 2 * n <= 4096) w *= 2;
+  return w;
+}
+inline size_t dyn_z_lds(int n, int lpw, size_t csz) { return 2 * (size_t)lpw * n * csz; }
+inline size_t dyn_s_lds(int n, int w, size_t csz) { return 2 * (size_t)w * n * csz; }
+
+template <typename T, int N>
+hipError_t launch_z_n(int mode, int sign, FftArgs<T>& a, int64_t grid, size_t shm,
+                      hipStream_t st) {
+  dim3 g((unsigned)grid), b(256);
+  switch (mode) {
+    case M_C2C:
+      if (sign < 0) hipLaunchKernelGGL((zfft_kernel<T, N, M_C2C, -1>), g, b, shm, st, a);
+      else hipLaunchKernelGGL((zfft_kernel<T, N, M_C2C, +1>), g, b, shm, st, a);
+      break;
+    case M_R2C: hipLaunchKernelGGL((zfft_kernel<T, N, M_R2C, -1>), g, b, shm, st, a); break;
+    case M_C2R: hipLaunchKernelGGL((zfft_kernel<T, N, M_C2R, +1>), g, b, shm, st, a); break;
+    case M_DERIV: hipLaunchKernelGGL((zfft_kernel<T, N, M_DERIV, -1>), g, b, shm, st, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template <typename T, int N>
+hipError_t launch_s_n(int mode, int sign, FftArgs<T>& a, int64_t grid, size_t shm,
+                      hipStream_t st) {
+  dim3 g((unsigned)grid), b(256);
+  switch (mode) {
+    case M_C2C:
+      if (sign < 0) hipLaunchKernelGGL((sfft_kernel<T, N, M_C2C, -1>), g, b, shm, st, a);
+      else hipLaunchKernelGGL((sfft_kernel<T, N, M_C2C, +1>), g, b, shm, st, a);
+      break;
+    case M_R2C: hipLaunchKernelGGL((sfft_kernel<T, N, M_R2C, -1>), g, b, shm, st, a); break;
+    case M_C2R: hipLaunchKernelGGL((sfft_kernel<T, N, M_C2R, +1>), g, b, shm, st, a); break;
+    case M_DERIV: hipLaunchKernelGGL((sfft_kernel<T, N, M_DERIV, -1>), g, b, shm, st, a); break;
+    case M_MULT: hipLaunchKernelGGL((sfft_kernel<T, N, M_MULT, -1>), g, b, shm, st, a); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// Raise the dynamic-LDS cap of every dynamic-engine kernel once.
+template <typename T>
+inline void dyn_lds_attrs() {
+  static bool done = false;
+  if (done) return;
+  done = true;
+  const int cap = 160 * 1024;
+  (void)hipFuncSetAttribute((const void*)zfft_kernel<T, 0, M_C2C, -1>, hipFuncAttributeMaxDynamicSharedMemorySize, cap);
+  (void)hipFuncSetAttribute((const void*)zfft_kernel<T, 0, M_C2C, +1>, hipFuncAttributeMaxDynamicSharedMemorySize, cap);
+  (void)hipFuncSetAttribute((const void*)zfft_kernel<T, 0, M_R2C, -1>, hipFuncAttributeMaxDynamicSharedMemorySize, cap);
+  (void)hipFuncSetAttribute((const void*)zfft_kernel<T, 0, M_C2R, +1>, hipFuncAttributeMaxDynamicSharedMemorySize, cap);
+  (void)hipFuncSetAttribute((const void*)zfft_kernel<T, 0, M_DERIV, -1>, hipFuncAttributeMaxDynamicSharedMemorySize, cap);
+  (void)hipFuncSetAttribute((const void*)sfft_kernel<T, 0, M_C2C, -1>, hipFuncAttributeMaxDynamicSharedMemorySize, cap);
+  (void)hipFuncSetAttribute((const void*)sfft_kernel<T, 0, M_C2C, +1>, hipFuncAttributeMaxDynamicSharedMemorySize, cap);
+  (void)hipFuncSetAttribute((const void*)sfft_kernel<T, 0, M_R2C, -1>, hipFuncAttributeMaxDynamicSharedMemorySize, cap);
+  (void)hipFuncSetAttribute((const void*)sfft_kernel<T, 0, M_C2R, +1>, hipFuncAttributeMaxDynamicSharedMemorySize, cap);
+  (void)hipFuncSetAttribute((const void*)sfft_kernel<T, 0, M_DERIV, -1>, hipFuncAttributeMaxDynamicSharedMemorySize, cap);
+  (void)hipFuncSetAttribute((const void*)sfft_kernel<T, 0, M_MULT, -1>, hipFuncAttributeMaxDynamicSharedMemorySize, cap);
+}
+
+// Launch one axis pass.  strided = false: contiguous lines (z layout).
+// Returns hipErrorNotSupported when the length does not fit one LDS tile.
+template <typename T>
+hipError_t launch_fft(bool strided, int mode, int sign, FftArgs<T> a, const AxisTab<T>& tab,
+                      hipStream_t st) {
+  const int n = tab.n;
+  a.sc = tab.sc;
+  a.sc.n = n;
+  const int64_t units = pair_mode(mode) ? (a.nlines + 1) / 2 : a.nlines;
+  if (units <= 0) return hipSuccess;
+  const size_t csz = sizeof(cpx<T>);
+  if (tab.is_static) {
+    a.tw = tab.tw_static;
+    int per;
+#define AFFT_CASE(NN)                                                              \
+  case NN:                                                                         \
+    per = strided ? SCfg<T, NN>::W : ZCfg<NN>::LPW;                                \
+    a.lpw = per;                                                                   \
+    return strided ? launch_s_n<T, NN>(mode, sign, a, (units + per - 1) / per, 0, st) \
+                   : launch_z_n<T, NN>(mode, sign, a, (units + per - 1) / per, 0, st);
+    switch (n) {
+      AFFT_CASE(16)
+      AFFT_CASE(32)
+      AFFT_CASE(64)
+      AFFT_CASE(128)
+      AFFT_CASE(256)
+      AFFT_CASE(512)
+      AFFT_CASE(1024)
+      AFFT_CASE(2048)
+      AFFT_CASE(4096)
+      default: return hipErrorInvalidValue;
+    }
+#undef AFFT_CASE
+  }
+  a.tw = tab.tw_full;
+  dyn_lds_attrs<T>();
+  if (strided) {
+    const int w = dyn_cols(n);
+    const size_t shm = dyn_s_lds(n, w, csz);
+    if (shm > 160 * 1024) return hipErrorNotSupported;
+    a.lpw = w;
+    return launch_s_n<T, 0>(mode, sign, a, (units + w - 1) / w, shm, st);
+  } else {
+    const int l = dyn_lines(n);
+    const size_t shm = dyn_z_lds(n, l, csz);
+    if (shm > 160 * 1024) return hipErrorNotSupported;
+    a.lpw = l;
+    return launch_z_n<T, 0>(mode, sign, a, (units + l - 1) / l, shm, st);
+  }
+}
+
+template <typename T>
+hipError_t launch_mult(const cpx<T>* in, cpx<T>* out, int kind, const long N[3],
+                       const long os[3], const long ost[3], double scale, hipStream_t st) {
+  const int64_t total = (int64_t)os[0] * os[1] * os[2];
+  if (total <= 0) return hipSuccess;
+  int64_t blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL((mult_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, st, in, out, kind,
+                     N[0], N[1], N[2], os[0], os[1], os[2], ost[0], ost[1], ost[2], scale);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_axpy(int64_t n, T alpha, const T* x, T* y, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL((axpy_kernel<T>), dim3((unsigned)blocks), dim3(256), 0, st, n, alpha, x, y);
+  return hipGetLastError();
+}
+
+}  // namespace afft

@@ -1,0 +1,123 @@
+// plan.hpp -- the plan object behind the C ABI (include/accfft_amd.h).
+//
+// Mirrors struct accfft_plan_gpu(f) (include/accfft_gpu.h:37-95) on a 1x1
+// grid: sizes, the data pointers recorded at plan time, the in-place flag,
+// and in place of cuFFT handles the device twiddle tables of each axis.
+// Scratch (the reference's Mem_Mgr_gpu, src/transpose_gpu.cpp:58-131) is a
+// pair of workspaces allocated on first use.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cmath>
+#include <vector>
+
+#include "../../include/accfft_amd.h"
+#include "fft_launch.hpp"
+
+struct accfft_amd_plan {
+  int n[3] = {0, 0, 0};
+  int fp = ACCFFT_AMD_F64;
+  int kind = 0;  // 1 = r2c, 2 = c2c
+  bool inplace = false;
+  unsigned flags = 0;
+  void* data = nullptr;
+  void* data_out = nullptr;
+  int64_t alloc_max = 0;
+  int isize[3] = {0, 0, 0}, istart[3] = {0, 0, 0};
+  int osize[3] = {0, 0, 0}, ostart[3] = {0, 0, 0};
+  afft::AxisTab<float> tf[3];
+  afft::AxisTab<double> td[3];
+  std::vector<void*> dev_mem;  // table allocations
+  void* ws[2] = {nullptr, nullptr};
+  size_t ws_bytes[2] = {0, 0};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  float last_ms = 0.f;
+  int device = 0;
+};
+
+namespace afft {
+
+// exp(-2 pi i m / n) in long double, rounded once to T.
+template <typename T>
+inline cpx<T> host_root(long m, long n) {
+  m %= n;
+  if (m < 0) m += n;
+  if ((4 * m) % n == 0) {
+    static const T c[4] = {1, 0, -1, 0}, s[4] = {0, -1, 0, 1};
+    const int q = (int)((4 * m) / n);
+    return {c[q], s[q]};
+  }
+  const long double a = 6.283185307179586476925286766559005768L * (long double)m / (long double)n;
+  return {(T)cosl(a), (T)(-sinl(a))};
+}
+
+inline int static_sched(int n, int R[3]) {
+#define AFFT_SCHED(NN)                                              \
+  case NN:                                                          \
+    for (int i = 0; i < 3; ++i) R[i] = Sched<NN>::R[i];             \
+    return Sched<NN>::S;
+  switch (n) {
+    AFFT_SCHED(16)
+    AFFT_SCHED(32)
+    AFFT_SCHED(64)
+    AFFT_SCHED(128)
+    AFFT_SCHED(256)
+    AFFT_SCHED(512)
+    AFFT_SCHED(1024)
+    AFFT_SCHED(2048)
+    AFFT_SCHED(4096)
+    default: return 0;
+  }
+#undef AFFT_SCHED
+}
+
+inline DynSched dyn_sched(int n) {
+  DynSched s{};
+  s.n = n;
+  s.nst = 0;
+  int m = n;
+  const int pref[] = {16, 8, 4, 2, 3, 5, 7};
+  for (int f : pref)
+    while (m % f == 0 && s.nst < 20) {
+      s.rad[s.nst++] = f;
+      m /= f;
+    }
+  for (int f = 11; (long)f * f <= m; f += 2)
+    while (m % f == 0 && s.nst < 20) {
+      s.rad[s.nst++] = f;
+      m /= f;
+    }
+  if (m > 1 && s.nst < 20) s.rad[s.nst++] = m;
+  if (n == 1) {
+    s.nst = 0;
+  }
+  return s;
+}
+
+// Builds the per-stage table of the static engine: for stage s >= 1 with
+// span NS and radix R, entries (r-1)*NS + k = exp(-2 pi i r k / (NS R)).
+template <typename T>
+inline std::vector<cpx<T>> static_table(int n) {
+  int R[3];
+  const int S = static_sched(n, R);
+  std::vector<cpx<T>> tw;
+  int ns = R[0];
+  for (int s = 1; s < S; ++s) {
+    for (int r = 1; r < R[s]; ++r)
+      for (int k = 0; k < ns; ++k) tw.push_back(host_root<T>((long)r * k, (long)ns * R[s]));
+    ns *= R[s];
+  }
+  if (tw.empty()) tw.push_back({(T)1, (T)0});
+  return tw;
+}
+
+template <typename T>
+inline std::vector<cpx<T>> full_table(int n) {
+  std::vector<cpx<T>> tw(n > 0 ? n : 1);
+  for (int m = 0; m < n; ++m) tw[m] = host_root<T>(m, n);
+  return tw;
+}
+
+}  // namespace afft

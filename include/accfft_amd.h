@@ -1,0 +1,190 @@
+/*
+ * accfft_amd.h -- C ABI of the MI355X (gfx950) implementation of AccFFT's
+ * single-GPU hot path.
+ *
+ * Plain C: opaque plan handle, explicit status codes, int64 sizes, device
+ * pointers as void*.  The drop-in C++ API (accfft_gpu.h, accfft_gpuf.h,
+ * accfft_operators_gpu.h, accfft_common.h in this directory) is a set of
+ * inline wrappers over these entry points.
+ *
+ * Each declaration names the reference interface it replaces (paths are
+ * relative to the reference tree).  "fp" selects the precision:
+ * ACCFFT_AMD_F64 replaces the double API (accfft_gpu.h), ACCFFT_AMD_F32 the
+ * float API (accfft_gpuf.h).
+ *
+ * Stream semantics: every call is ordered on the plan's stream (default:
+ * the null stream) and returns without waiting unless the plan was created
+ * with ACCFFT_AMD_BLOCKING or a timer pointer is passed (then the call
+ * synchronises to read its events, as the reference does at
+ * src/accfft_gpu.cpp:478-487).
+ */
+#ifndef ACCFFT_AMD_H
+#define ACCFFT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct accfft_amd_plan accfft_amd_plan;
+
+/* Status codes (the reference returns NULL / -1 / prints; see
+ * src/accfft_gpu.cpp:186-190, :641-646 and src/accfft_common.cpp:179-184). */
+enum {
+  ACCFFT_AMD_OK = 0,
+  ACCFFT_AMD_ERR_INVALID = 1,     /* bad argument (size <= 0, bad axis, ...) */
+  ACCFFT_AMD_ERR_OVERFLOW = 2,    /* a size does not fit the target type */
+  ACCFFT_AMD_ERR_UNSUPPORTED = 3, /* length too large for one LDS tile */
+  ACCFFT_AMD_ERR_NOT_BAKED = 4,   /* plan of the wrong kind for this call */
+  ACCFFT_AMD_ERR_HIP = 5,         /* a HIP runtime call failed */
+  ACCFFT_AMD_ERR_ALLOC = 6,       /* device allocation failed */
+  ACCFFT_AMD_ERR_NULL = 7         /* NULL plan or pointer */
+};
+
+enum { ACCFFT_AMD_F64 = 0, ACCFFT_AMD_F32 = 1 };
+
+/* Plan flags.  ACCFFT_ESTIMATE(1)/MEASURE(2)/PATIENT(4) of
+ * include/accfft_common.h:40-42 are accepted and do not change results. */
+#define ACCFFT_AMD_BLOCKING 0x100u       /* synchronise the stream after each call */
+#define ACCFFT_AMD_PRESERVE_INPUT 0x200u /* c2r / inverse c2c keep their input */
+
+/* Directions (include/accfft_common.h:37-39). */
+#define ACCFFT_AMD_FORWARD (-1)
+#define ACCFFT_AMD_BACKWARD (+1)
+
+const char* accfft_amd_status_string(int status);
+int accfft_amd_version(void);
+
+/* ---------------------------------------------------------------- sizes */
+/* accfft_local_size_dft_r2c_gpu(f) (src/accfft_gpu.cpp:62-65,
+ * src/accfft_gpuf.cpp:59-62) -> accfft_local_size_dft_r2c_t
+ * (src/accfft_common.cpp:98-192) on a p0 x p1 grid at coords (c0, c1).
+ * alloc_max is returned in bytes as int64 (the reference returns int). */
+int accfft_amd_local_size_dft_r2c(const int n[3], int isize[3], int istart[3],
+                                  int osize[3], int ostart[3], int fp, int p0,
+                                  int p1, int c0, int c1, int64_t* alloc_max);
+/* accfft_local_size_dft_c2c_gpu(f) (src/accfft_gpu.cpp:689-692) ->
+ * accfft_local_size_dft_c2c_t (src/accfft_common.cpp:203-256). */
+int accfft_amd_local_size_dft_c2c(const int n[3], int isize[3], int istart[3],
+                                  int osize[3], int ostart[3], int fp, int p0,
+                                  int p1, int c0, int c1, int64_t* alloc_max);
+
+/* ---------------------------------------------------------------- plans */
+/* accfft_plan_dft_3d_r2c_gpu(f) (src/accfft_gpu.cpp:77-442;
+ * src/accfft_gpuf.cpp:74-441).  In place iff data_out == data (rows padded
+ * to 2*(n[2]/2+1) reals).  1x1 process grid. */
+int accfft_amd_plan_dft_3d_r2c(const int n[3], void* data, void* data_out, int fp,
+                               unsigned flags, accfft_amd_plan** plan);
+/* accfft_plan_dft_3d_c2c_gpu(f) (src/accfft_gpu.cpp:704-931). */
+int accfft_amd_plan_dft_3d_c2c(const int n[3], void* data, void* data_out, int fp,
+                               unsigned flags, accfft_amd_plan** plan);
+/* accfft_destroy_plan_gpu(f) (src/accfft_gpu.cpp:1150-1192).  Also frees the
+ * plan object itself (the reference leaks it). */
+int accfft_amd_destroy_plan(accfft_amd_plan* plan);
+/* Stream for all later calls on this plan (a hipStream_t, NULL = null stream). */
+int accfft_amd_plan_set_stream(accfft_amd_plan* plan, void* stream);
+/* Plan facts the drop-in wrappers mirror into accfft_plan_gpu(f):
+ * n, isize, istart, osize, ostart (3 each), alloc_max, fp, kind (1 = r2c,
+ * 2 = c2c), inplace. */
+int accfft_amd_plan_info(const accfft_amd_plan* plan, int n[3], int isize[3],
+                         int istart[3], int osize[3], int ostart[3],
+                         int64_t* alloc_max, int* fp, int* kind, int* inplace);
+/* Default data pointers recorded at plan time (src/accfft_gpu.cpp:447-450). */
+int accfft_amd_plan_data(const accfft_amd_plan* plan, void** data, void** data_out);
+
+/* ---------------------------------------------------------------- execute */
+/* accfft_execute_r2c_gpu(f) (src/accfft_gpu.cpp:635-653) and
+ * accfft_execute_gpu(dir=-1) (:444-539).  xyz bit d selects axis d (bit 0 =
+ * x), as std::bitset<3>.  NULL pointers take the plan's pointers.  timer[4]
+ * accumulates the FFT time in seconds (doc/timing.md). */
+int accfft_amd_execute_r2c(accfft_amd_plan* plan, const void* in, void* out,
+                           double* timer, unsigned xyz);
+/* accfft_execute_c2r_gpu(f) (:655-675), accfft_execute_gpu(dir=+1)
+ * (:540-603).  Overwrites its input unless ACCFFT_AMD_PRESERVE_INPUT. */
+int accfft_amd_execute_c2r(accfft_amd_plan* plan, void* in, void* out,
+                           double* timer, unsigned xyz);
+/* accfft_execute_c2c_gpu(f) (:943-1136). */
+int accfft_amd_execute_c2c(accfft_amd_plan* plan, int direction, void* in,
+                           void* out, double* timer, unsigned xyz);
+/* Per-axis transforms accfft_execute_{r2c,c2r}_{z,y,x}_gpu_t
+ * (:1212-1604).  axis 2 (z): real [N0][N1][N2] <-> complex [N0][N1][N2/2+1];
+ * axis 1 (y): <-> [N0][N1/2+1][N2]; axis 0 (x): <-> [N0/2+1][N1][N2].  The
+ * input is never overwritten. */
+int accfft_amd_execute_r2c_axis(accfft_amd_plan* plan, int axis, const void* in,
+                                void* out, double* timer);
+int accfft_amd_execute_c2r_axis(accfft_amd_plan* plan, int axis, const void* in,
+                                void* out, double* timer);
+
+/* ---------------------------------------------------------------- operators */
+/* accfft_grad_gpu(f) (src/operators_gpu.cpp:85-88; operators_gpu.txx:172-278):
+ * component d = c2r_d(i*k_d/N_d * r2c_d(A)), one fused pass per component.
+ * mask bit d selects the component (pXYZ); NULL outputs are skipped.
+ * slow != 0 gives accfft_grad_gpu_slow_t (:72-170). */
+int accfft_amd_grad(accfft_amd_plan* plan, void* Ax, void* Ay, void* Az,
+                    const void* A, unsigned mask, double* timer, int slow);
+/* accfft_laplace_gpu(f) (operators_gpu.txx:280-334). */
+int accfft_amd_laplace(accfft_amd_plan* plan, void* LA, const void* A, double* timer);
+/* accfft_biharmonic_gpu(f) (operators_gpu.txx:555-609). */
+int accfft_amd_biharmonic(accfft_amd_plan* plan, void* BA, const void* A, double* timer);
+/* Inverse Laplacian: CPU-only upstream (src/operators.txx:276-333,
+ * :1270-1321); here on the GPU. */
+int accfft_amd_inv_laplace(accfft_amd_plan* plan, void* out, const void* A, double* timer);
+/* Inverse biharmonic: CPU-only upstream (src/operators.txx:336-392). */
+int accfft_amd_inv_biharmonic(accfft_amd_plan* plan, void* out, const void* A, double* timer);
+/* accfft_divergence_gpu(f) (operators_gpu.txx:448-553); slow != 0 gives
+ * accfft_divergence_gpu_slow_t (:336-446). */
+int accfft_amd_divergence(accfft_amd_plan* plan, void* divA, const void* Ax,
+                          const void* Ay, const void* Az, double* timer, int slow);
+
+/* Element-wise spectral multiply, the extern "C" multipliers of
+ * src/operators_cuda.cu:421-486 (grad_mult_wave_number{x,y,z}_gpu_c(f),
+ * laplace_mult_wave_number_gpu_c(f), biharmonic_mult_wave_number_gpu_c(f))
+ * plus the inverse symbols.  kind: 0/1/2 grad x/y/z, 3 laplace,
+ * 4 biharmonic, 5 inverse laplace, 6 inverse biharmonic.  scale_xyz selects
+ * the axes whose lengths divide the result (the grad launchers' bitset,
+ * operators_cuda.cu:219-229; the other kinds use all three). */
+int accfft_amd_mult_wave_number(int kind, int fp, void* wA, const void* A,
+                                const int n[3], const int osize[3],
+                                const int ostart[3], unsigned scale_xyz,
+                                void* stream);
+/* daxpy_gpu_c(f) (operators_cuda.cu:470-474): y += alpha * x. */
+int accfft_amd_axpy(int fp, int64_t n, double alpha, const void* x, void* y,
+                    void* stream);
+
+/* ---------------------------------------------------------------- transposes */
+/* Local (on-GPU) transposes of src/transpose_cuda.cu, bit-exact
+ * permutations of tuples of elem_size-byte elements.
+ * local_transpose_cuda(r,c,t,in,out) (:242-257, kernel cuda1 :6-11):
+ *   [r][c][t] -> [c][r][t]. */
+int accfft_amd_local_transpose(int r, int c, int64_t t, const void* in, void* out,
+                               int elem_size, void* stream);
+/* local_transpose_cuda(r,c,t,t2,in,out) (:207-224, cuda2 :13-21): unpack with
+ * a ragged last row of t2-element tuples. */
+int accfft_amd_local_transpose_ragged(int r, int c, int64_t t, int64_t t2,
+                                      const void* in, void* out, int elem_size,
+                                      void* stream);
+/* local_transpose_col_cuda(r,c,t,t2,in,out) (:78-95, cuda4 :59-66): pack
+ * with a ragged last column. */
+int accfft_amd_local_transpose_col(int r, int c, int64_t t, int64_t t2,
+                                   const void* in, void* out, int elem_size,
+                                   void* stream);
+/* local_transpose_cuda(r,c,t,A) (:227-239, cuda3 :23-57): in place,
+ * cycle-following. */
+int accfft_amd_local_transpose_inplace(int r, int c, int64_t t, void* A,
+                                       int elem_size, void* stream);
+
+/* ---------------------------------------------------------------- misc */
+/* Synchronise the plan's stream (what the reference's blocking calls do at
+ * src/accfft_gpu.cpp:620). */
+int accfft_amd_plan_sync(accfft_amd_plan* plan);
+/* Time of the last kernel launched by this plan's most recent call with a
+ * timer, for benchmarking: device milliseconds of [first, last] launch. */
+int accfft_amd_last_elapsed_ms(const accfft_amd_plan* plan, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ACCFFT_AMD_H */
