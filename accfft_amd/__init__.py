@@ -85,6 +85,11 @@ def lib() -> ctypes.CDLL:
     L.accfft_amd_local_transpose_inplace.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64,
                                                      _vp, ctypes.c_int, _vp]
     L.accfft_amd_last_elapsed_ms.argtypes = [_vp, ctypes.POINTER(ctypes.c_float)]
+    L.accfft_amd_plan_set_profiling.argtypes = [_vp, ctypes.c_int]
+    L.accfft_amd_profile_reset.argtypes = [_vp]
+    L.accfft_amd_profile_count.argtypes = [_vp, ctypes.POINTER(ctypes.c_int)]
+    L.accfft_amd_profile_get.argtypes = [_vp, ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                                         dp, i64p, dp]
     _lib = L
     return L
 
@@ -139,7 +144,11 @@ def local_size_dft_c2c(n: Sequence[int], precision: int = F64, grid=(1, 1), coor
 
 # ------------------------------------------------------------------ plan
 class Plan:
-    """accfft_plan_gpu(f) on a 1x1 grid (include/accfft_gpu.h:37-95)."""
+    """accfft_plan_gpu(f) on a 1x1 grid (include/accfft_gpu.h:37-95).
+
+    As upstream (src/accfft_gpu.cpp:99-103) the plan is in place -- real rows
+    padded to 2*(n[2]//2+1) -- exactly when ``data`` and ``data_out`` are the
+    same buffer, which includes passing neither."""
 
     def __init__(self, kind: str, n: Sequence[int], data=None, data_out=None,
                  precision: int = F64, flags: int = ACCFFT_MEASURE):
@@ -259,6 +268,25 @@ class Plan:
 
     def sync(self):
         _check(lib().accfft_amd_plan_sync(self._bind()), "sync")
+
+    def set_profiling(self, enable: bool = True):
+        _check(lib().accfft_amd_plan_set_profiling(self._h, int(enable)), "set_profiling")
+
+    def profile_reset(self):
+        _check(lib().accfft_amd_profile_reset(self._h), "profile_reset")
+
+    def profile(self) -> dict:
+        """{kernel kind: (total_ms, launches, algorithmic bytes per launch)}."""
+        n = ctypes.c_int()
+        _check(lib().accfft_amd_profile_count(self._h, ctypes.byref(n)), "profile_count")
+        out = {}
+        for i in range(n.value):
+            buf = ctypes.create_string_buffer(128)
+            ms, cnt, b = ctypes.c_double(), ctypes.c_int64(), ctypes.c_double()
+            _check(lib().accfft_amd_profile_get(self._h, i, buf, 128, ctypes.byref(ms),
+                                                ctypes.byref(cnt), ctypes.byref(b)), "profile_get")
+            out[buf.value.decode()] = (ms.value, cnt.value, b.value)
+        return out
 
     def last_elapsed_ms(self) -> float:
         v = ctypes.c_float()

@@ -107,6 +107,11 @@ void free_plan(accfft_amd_plan* p) {
     if (p->ws[i]) (void)hipFree(p->ws[i]);
   if (p->ev0) (void)hipEventDestroy(p->ev0);
   if (p->ev1) (void)hipEventDestroy(p->ev1);
+  for (auto& q : p->prof_pending) {
+    (void)hipEventDestroy(q.a);
+    (void)hipEventDestroy(q.b);
+  }
+  for (hipEvent_t e : p->prof_pool) (void)hipEventDestroy(e);
   delete p;
 }
 
@@ -396,6 +401,48 @@ int accfft_amd_axpy(int fp, int64_t n, double alpha, const void* x, void* y, voi
 int accfft_amd_plan_sync(accfft_amd_plan* p) {
   if (!p) return ACCFFT_AMD_ERR_NULL;
   return hipStreamSynchronize(p->stream) == hipSuccess ? ACCFFT_AMD_OK : ACCFFT_AMD_ERR_HIP;
+}
+
+int accfft_amd_plan_set_profiling(accfft_amd_plan* p, int enable) {
+  if (!p) return ACCFFT_AMD_ERR_NULL;
+  if (!enable) {
+    int st = afft::prof_collect(p);
+    p->prof = false;
+    return st;
+  }
+  p->prof = true;
+  return ACCFFT_AMD_OK;
+}
+
+int accfft_amd_profile_reset(accfft_amd_plan* p) {
+  if (!p) return ACCFFT_AMD_ERR_NULL;
+  int st = afft::prof_collect(p);
+  p->prof_kinds.clear();
+  return st;
+}
+
+int accfft_amd_profile_count(accfft_amd_plan* p, int* nkinds) {
+  if (!p || !nkinds) return ACCFFT_AMD_ERR_NULL;
+  int st = afft::prof_collect(p);
+  *nkinds = (int)p->prof_kinds.size();
+  return st;
+}
+
+int accfft_amd_profile_get(accfft_amd_plan* p, int idx, char* name, int name_len,
+                           double* total_ms, int64_t* launches, double* bytes_per_launch) {
+  if (!p) return ACCFFT_AMD_ERR_NULL;
+  int st = afft::prof_collect(p);
+  if (st) return st;
+  if (idx < 0 || idx >= (int)p->prof_kinds.size()) return ACCFFT_AMD_ERR_INVALID;
+  const auto& k = p->prof_kinds[idx];
+  if (name && name_len > 0) {
+    strncpy(name, k.name.c_str(), (size_t)name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  if (total_ms) *total_ms = k.total_ms;
+  if (launches) *launches = k.launches;
+  if (bytes_per_launch) *bytes_per_launch = k.bytes;
+  return ACCFFT_AMD_OK;
 }
 
 int accfft_amd_last_elapsed_ms(const accfft_amd_plan* p, float* ms) {

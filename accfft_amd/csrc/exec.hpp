@@ -10,6 +10,8 @@
 // no-ops at P = 1 (src/transpose_gpu.cpp:3819-3821, :1164-1179) and are
 // not issued.
 #pragma once
+#include <stdio.h>
+
 #include "plan.hpp"
 
 namespace afft {
@@ -41,6 +43,56 @@ inline int ensure_ws(accfft_amd_plan* p, int i, size_t bytes) {
   }
   p->ws_bytes[i] = bytes;
   return ACCFFT_AMD_OK;
+}
+
+template <typename T>
+inline const char* mode_name(int mode, int sign) {
+  switch (mode) {
+    case M_C2C: return sign < 0 ? "c2c_fwd" : "c2c_inv";
+    case M_R2C: return "r2c";
+    case M_C2R: return "c2r";
+    case M_MULT: return "mult";
+    default: return "deriv";
+  }
+}
+
+// Launches one pass; with profiling on, brackets it with events labelled
+// "<z|s>fft_<mode>_n<N>_<tag>" and records its algorithmic bytes
+// (elements read + elements written, each once).
+template <typename T>
+inline int run_pass(accfft_amd_plan* p, const char* tag, bool strided, int mode, int sign,
+                    const FftArgs<T>& a, const AxisTab<T>& tab) {
+  if (!p->prof) return status_of(launch_fft<T>(strided, mode, sign, a, tab, p->stream));
+  const int64_t n = tab.n;
+  const int64_t nh = n / 2 + 1;
+  const double rs = sizeof(T), cs = 2.0 * sizeof(T);
+  double bytes;
+  switch (mode) {
+    case M_R2C: bytes = a.nlines * (n * rs + nh * cs); break;
+    case M_C2R: bytes = a.nlines * (nh * cs + n * rs); break;
+    case M_DERIV: bytes = a.nlines * (2.0 * n * rs) * (a.accumulate ? 1.5 : 1.0); break;
+    default: bytes = a.nlines * (2.0 * n * cs); break;
+  }
+  char name[96];
+  snprintf(name, sizeof(name), "%cfft_%s_n%d_%s_%s", strided ? 's' : 'z', mode_name<T>(mode, sign),
+           (int)n, tag, sizeof(T) == 4 ? "f32" : "f64");
+  int kind = -1;
+  for (size_t i = 0; i < p->prof_kinds.size(); ++i)
+    if (p->prof_kinds[i].name == name) kind = (int)i;
+  if (kind < 0) {
+    accfft_amd_plan::ProfKind k;
+    k.name = name;
+    p->prof_kinds.push_back(k);
+    kind = (int)p->prof_kinds.size() - 1;
+  }
+  p->prof_kinds[kind].bytes = bytes;
+  hipEvent_t e0 = prof_event(p), e1 = prof_event(p);
+  (void)hipEventRecord(e0, p->stream);
+  const int st = status_of(launch_fft<T>(strided, mode, sign, a, tab, p->stream));
+  (void)hipEventRecord(e1, p->stream);
+  p->prof_pending.push_back({kind, e0, e1});
+  if (p->prof_pending.size() > 4096) return st ? st : prof_collect(p);
+  return st;
 }
 
 template <typename T>
@@ -79,12 +131,12 @@ struct Ops {
   static int z_r2c(accfft_amd_plan* p, const void* in, void* out, bool padded) {
     const int64_t N0 = p->n[0], N1 = p->n[1], N2 = p->n[2], Nh = N2 / 2 + 1;
     auto a = zargs(in, out, N0 * N1, padded ? 2 * Nh : N2, Nh);
-    return status_of(launch_fft<T>(false, M_R2C, -1, a, tab(p, 2), p->stream));
+    return run_pass<T>(p, "z", false, M_R2C, -1, a, tab(p, 2));
   }
   static int z_c2r(accfft_amd_plan* p, const void* in, void* out, bool padded) {
     const int64_t N0 = p->n[0], N1 = p->n[1], N2 = p->n[2], Nh = N2 / 2 + 1;
     auto a = zargs(in, out, N0 * N1, Nh, padded ? 2 * Nh : N2);
-    return status_of(launch_fft<T>(false, M_C2R, +1, a, tab(p, 2), p->stream));
+    return run_pass<T>(p, "z", false, M_C2R, +1, a, tab(p, 2));
   }
   // y pass over a complex [N0][N1][L] array (L = Nh for r2c, N2 for c2c)
   static FftArgs<T> y_args(const accfft_amd_plan* p, const void* in, void* out, int64_t L) {
@@ -97,10 +149,10 @@ struct Ops {
     return sargs(in, out, N1 * L, N1 * L, 0, 0, N1 * L, N1 * L);
   }
   static int y_c2c(accfft_amd_plan* p, const void* in, void* out, int64_t L, int sign) {
-    return status_of(launch_fft<T>(true, M_C2C, sign, y_args(p, in, out, L), tab(p, 1), p->stream));
+    return run_pass<T>(p, "y", true, M_C2C, sign, y_args(p, in, out, L), tab(p, 1));
   }
   static int x_c2c(accfft_amd_plan* p, const void* in, void* out, int64_t L, int sign) {
-    return status_of(launch_fft<T>(true, M_C2C, sign, x_args(p, in, out, L), tab(p, 0), p->stream));
+    return run_pass<T>(p, "x", true, M_C2C, sign, x_args(p, in, out, L), tab(p, 0));
   }
   // forward x, symbol, inverse x, fused (rows along x)
   static int x_mult(accfft_amd_plan* p, const void* in, void* out, int64_t L, int sym,
@@ -113,7 +165,7 @@ struct Ops {
     a.N2 = p->n[2];
     a.Ch = L;
     a.maxis = 0;
-    return status_of(launch_fft<T>(true, M_MULT, -1, a, tab(p, 0), p->stream));
+    return run_pass<T>(p, "x", true, M_MULT, -1, a, tab(p, 0));
   }
   static int y_mult(accfft_amd_plan* p, const void* in, void* out, int64_t L, int sym,
                     double scale) {
@@ -125,7 +177,7 @@ struct Ops {
     a.N2 = p->n[2];
     a.Ch = L;
     a.maxis = 1;
-    return status_of(launch_fft<T>(true, M_MULT, -1, a, tab(p, 1), p->stream));
+    return run_pass<T>(p, "y", true, M_MULT, -1, a, tab(p, 1));
   }
   // fused per-axis derivative of a real field (operators_gpu.txx:172-278)
   static int deriv(accfft_amd_plan* p, int axis, const void* in, void* out, int accumulate) {
@@ -142,7 +194,7 @@ struct Ops {
     }
     a.scale = 1.0 / (double)p->n[axis];
     a.accumulate = accumulate;
-    return status_of(launch_fft<T>(strided, M_DERIV, -1, a, tab(p, axis), p->stream));
+    return run_pass<T>(p, axis == 0 ? "x" : axis == 1 ? "y" : "z", strided, M_DERIV, -1, a, tab(p, axis));
   }
 
   // ---------------------------------------------------------- public paths
@@ -191,7 +243,7 @@ struct Ops {
       void* o = out;
       if (xyz & 4) {
         auto a = zargs(in, out, N0 * N1, N2, N2);
-        if ((st = status_of(launch_fft<T>(false, M_C2C, -1, a, tab(p, 2), p->stream)))) return st;
+        if ((st = run_pass<T>(p, "z", false, M_C2C, -1, a, tab(p, 2)))) return st;
       } else {
         o = in;
       }
@@ -217,7 +269,7 @@ struct Ops {
     }
     if (xyz & 4) {
       auto a = zargs(cur, out, N0 * N1, N2, N2);
-      return status_of(launch_fft<T>(false, M_C2C, +1, a, tab(p, 2), p->stream));
+      return run_pass<T>(p, "z", false, M_C2C, +1, a, tab(p, 2));
     }
     return ACCFFT_AMD_OK;
   }
@@ -229,7 +281,7 @@ struct Ops {
     FftArgs<T> a;
     if (axis == 1) a = sargs(in, out, N0 * N2, N2, N1 * N2, (N1 / 2 + 1) * N2, N2, N2);
     else a = sargs(in, out, N1 * N2, N1 * N2, 0, 0, N1 * N2, N1 * N2);
-    return status_of(launch_fft<T>(true, M_R2C, -1, a, tab(p, axis), p->stream));
+    return run_pass<T>(p, axis == 0 ? "x" : "y", true, M_R2C, -1, a, tab(p, axis));
   }
   static int c2r_axis(accfft_amd_plan* p, int axis, const void* in, void* out) {
     const int64_t N0 = p->n[0], N1 = p->n[1], N2 = p->n[2];
@@ -237,7 +289,7 @@ struct Ops {
     FftArgs<T> a;
     if (axis == 1) a = sargs(in, out, N0 * N2, N2, (N1 / 2 + 1) * N2, N1 * N2, N2, N2);
     else a = sargs(in, out, N1 * N2, N1 * N2, 0, 0, N1 * N2, N1 * N2);
-    return status_of(launch_fft<T>(true, M_C2R, +1, a, tab(p, axis), p->stream));
+    return run_pass<T>(p, axis == 0 ? "x" : "y", true, M_C2R, +1, a, tab(p, axis));
   }
 
   // ---------------------------------------------------------- operators

@@ -80,6 +80,26 @@ __device__ __forceinline__ void str(void* p, int64_t i, T v) {
   reinterpret_cast<T*>(p)[i] = v;
 }
 
+// Copies the static engine's twiddle table into LDS with an unrolled loop
+// (issued after the caller's data loads so that all loads overlap).
+template <typename T, int TWN, int TPB>
+__device__ __forceinline__ void stage_twiddles(cpx<T>* dst, const cpx<T>* __restrict__ src,
+                                               int t) {
+  if constexpr (TWN > 0) {
+    cpx<T> v[(TWN + TPB - 1) / TPB];
+#pragma unroll
+    for (int j = 0; j < (TWN + TPB - 1) / TPB; ++j) {
+      const int i = t + j * TPB;
+      if (i < TWN) v[j] = src[i];
+    }
+#pragma unroll
+    for (int j = 0; j < (TWN + TPB - 1) / TPB; ++j) {
+      const int i = t + j * TPB;
+      if (i < TWN) dst[i] = v[j];
+    }
+  }
+}
+
 // -------------------------------------------------------------- engines
 // N > 0: static engine; N == 0: dynamic engine.
 template <typename T, int N, int NT, int SIGN>
@@ -134,7 +154,6 @@ __global__ void __launch_bounds__(N > 0 ? ZCfg<(N > 0 ? N : 16)>::TPB : 256)
     constexpr int LPW = ZCfg<NS_>::LPW;
     __shared__ cpx<T> s_buf[LPW * NS_];
     __shared__ cpx<T> s_tw[TWN > 0 ? TWN : 1];
-    for (int i = t; i < TWN; i += TPB) s_tw[i] = a.tw[i];
     bufA = s_buf;
     bufB = s_buf;
     twl = s_tw;
@@ -149,7 +168,37 @@ __global__ void __launch_bounds__(N > 0 ? ZCfg<(N > 0 ? N : 16)>::TPB : 256)
   const int64_t nl = a.nlines;
 
   // ---------------- load
-  if constexpr (MODE == M_C2C || MODE == M_MULT) {
+  if constexpr (STATIC) {
+    // every thread stages its elements in registers so that all of its
+    // global loads are in flight together, then fills LDS
+    constexpr int LPW = ZCfg<NS_>::LPW;
+    constexpr int EP = LPW * NS_ / TPB;  // elements per thread
+    constexpr int NH = NS_ / 2 + 1;
+    cpx<T> v[EP];
+#pragma unroll
+    for (int j = 0; j < EP; ++j) {
+      const int e = t + j * TPB;
+      const int p = e / NS_, i = e % NS_;
+      if constexpr (MODE == M_C2C || MODE == M_MULT) {
+        const int64_t g = l0 + p;
+        v[j] = g < nl ? ldc<T>(a.in, g * a.idist + i) : cpx<T>{(T)0, (T)0};
+      } else if constexpr (MODE == M_R2C || MODE == M_DERIV) {
+        const int64_t ra = 2 * (l0 + p), rb = ra + 1;
+        v[j].x = ra < nl ? ldr<T>(a.in, ra * a.idist + i) : (T)0;
+        v[j].y = rb < nl ? ldr<T>(a.in, rb * a.idist + i) : (T)0;
+      } else {  // M_C2R: Z = A + iB from the two half spectra (Hermitian extension)
+        const int64_t ga = 2 * (l0 + p), gb = ga + 1;
+        const int k = i < NH ? i : NS_ - i;
+        cpx<T> A = ga < nl ? ldc<T>(a.in, ga * a.idist + k) : cpx<T>{(T)0, (T)0};
+        cpx<T> B = gb < nl ? ldc<T>(a.in, gb * a.idist + k) : cpx<T>{(T)0, (T)0};
+        if (k == 0 || 2 * k == NS_) { A.y = (T)0; B.y = (T)0; }
+        v[j] = i < NH ? cpx<T>{A.x - B.y, A.y + B.x} : cpx<T>{A.x + B.y, B.x - A.y};
+      }
+    }
+    stage_twiddles<T, TWN, TPB>(const_cast<cpx<T>*>(twl), a.tw, t);
+#pragma unroll
+    for (int j = 0; j < EP; ++j) bufA[t + j * TPB] = v[j];
+  } else if constexpr (MODE == M_C2C || MODE == M_MULT) {
     for (int e = t; e < lpw * n; e += TPB) {
       const int p = e / n, i = e - p * n;
       const int64_t g = l0 + p;
@@ -166,7 +215,7 @@ __global__ void __launch_bounds__(N > 0 ? ZCfg<(N > 0 ? N : 16)>::TPB : 256)
       if (rb < nl) v.y = ldr<T>(a.in, rb * a.idist + i);
       bufA[e] = v;
     }
-  } else {  // M_C2R: Z = A + iB from the two half spectra (Hermitian extension)
+  } else {  // M_C2R
     for (int e = t; e < lpw * n; e += TPB) {
       const int p = e / n, i = e - p * n;
       const int64_t ga = 2 * (l0 + p), gb = ga + 1;
@@ -190,9 +239,12 @@ __global__ void __launch_bounds__(N > 0 ? ZCfg<(N > 0 ? N : 16)>::TPB : 256)
     res = zrun<T, N, NT, +1>(bufA, bufB, a, twl, lpw);
   } else {  // M_DERIV and M_MULT (z has no MULT use; treat as DERIV-style symbol)
     res = zrun<T, N, NT, -1>(bufA, bufB, a, twl, lpw);
-    for (int e = t; e < lpw * n; e += TPB) {
-      const int p = e / n, i = e - p * n;
-      (void)p;
+    const int epm = STATIC ? ZCfg<NS_>::LPW * NS_ / TPB : (lpw * n + TPB - 1) / TPB;
+#pragma unroll
+    for (int j = 0; j < epm; ++j) {
+      const int e = t + j * TPB;
+      if (!STATIC && e >= lpw * n) break;
+      const int i = e % n;
       const T f = (T)(a.scale * (double)wave_sym(i, n));
       const cpx<T> v = res[e];
       res[e] = {-f * v.y, f * v.x};
@@ -203,14 +255,25 @@ __global__ void __launch_bounds__(N > 0 ? ZCfg<(N > 0 ? N : 16)>::TPB : 256)
   }
 
   // ---------------- store
+  // static: trip counts are compile-time so the loops unroll
+  constexpr int EPS = STATIC ? ZCfg<NS_>::LPW * NS_ / TPB : 1;
+  constexpr int EPH = STATIC ? (2 * ZCfg<NS_>::LPW * (NS_ / 2 + 1) + TPB - 1) / TPB : 1;
+  const int ep = STATIC ? EPS : (lpw * n + TPB - 1) / TPB;
+  const int eph = STATIC ? EPH : (2 * lpw * nh + TPB - 1) / TPB;
   if constexpr (MODE == M_C2C || MODE == M_MULT) {
-    for (int e = t; e < lpw * n; e += TPB) {
+#pragma unroll
+    for (int j = 0; j < ep; ++j) {
+      const int e = t + j * TPB;
+      if (!STATIC && e >= lpw * n) break;
       const int p = e / n, i = e - p * n;
       const int64_t g = l0 + p;
       if (g < nl) stc<T>(a.out, g * a.odist + i, res[e]);
     }
   } else if constexpr (MODE == M_R2C) {
-    for (int e = t; e < 2 * lpw * nh; e += TPB) {
+#pragma unroll
+    for (int j = 0; j < eph; ++j) {
+      const int e = t + j * TPB;
+      if (e >= 2 * lpw * nh) break;
       const int rl = e / nh, k = e - rl * nh;
       const int p = rl >> 1;
       const int64_t g = 2 * l0 + rl;
@@ -224,7 +287,10 @@ __global__ void __launch_bounds__(N > 0 ? ZCfg<(N > 0 ? N : 16)>::TPB : 256)
       stc<T>(a.out, g * a.odist + k, o);
     }
   } else {  // real outputs
-    for (int e = t; e < lpw * n; e += TPB) {
+#pragma unroll
+    for (int j = 0; j < ep; ++j) {
+      const int e = t + j * TPB;
+      if (!STATIC && e >= lpw * n) break;
       const int p = e / n, i = e - p * n;
       const int64_t ra = 2 * (l0 + p), rb = ra + 1;
       const cpx<T> z = res[e];
@@ -262,7 +328,6 @@ __global__ void __launch_bounds__(256) sfft_kernel(FftArgs<T> a) {
   if constexpr (STATIC) {
     __shared__ cpx<T> s_buf[WS * NS_];
     __shared__ cpx<T> s_tw[TWN > 0 ? TWN : 1];
-    for (int i = t; i < TWN; i += TPB) s_tw[i] = a.tw[i];
     bufA = s_buf;
     bufB = s_buf;
     twl = s_tw;
@@ -291,7 +356,33 @@ __global__ void __launch_bounds__(256) sfft_kernel(FftArgs<T> a) {
   const int64_t oa = (ga / a.C) * a.Dout + (ga % a.C), ob = (gb / a.C) * a.Dout + (gb % a.C);
 
   // ---------------- load
-  if constexpr (MODE == M_C2C || MODE == M_MULT) {
+  // rows handled by this thread: r = r0 + j*rstep, j < rp
+  constexpr int RP = STATIC ? NS_ * WS / TPB : 1;
+  const int rp = STATIC ? RP : (n - r0 + rstep - 1) / rstep;
+  if constexpr (STATIC) {
+    constexpr int RS = TPB / WS;
+    constexpr int NH = NS_ / 2 + 1;
+    cpx<T> v[RP];
+#pragma unroll
+    for (int j = 0; j < RP; ++j) {
+      const int r = r0 + j * RS;
+      if constexpr (MODE == M_C2C || MODE == M_MULT) {
+        v[j] = va ? ldc<T>(a.in, ia + (int64_t)r * a.Sin) : cpx<T>{(T)0, (T)0};
+      } else if constexpr (MODE == M_R2C || MODE == M_DERIV) {
+        v[j].x = va ? ldr<T>(a.in, ia + (int64_t)r * a.Sin) : (T)0;
+        v[j].y = vb ? ldr<T>(a.in, ib + (int64_t)r * a.Sin) : (T)0;
+      } else {
+        const int k = r < NH ? r : NS_ - r;
+        cpx<T> A = va ? ldc<T>(a.in, ia + (int64_t)k * a.Sin) : cpx<T>{(T)0, (T)0};
+        cpx<T> B = vb ? ldc<T>(a.in, ib + (int64_t)k * a.Sin) : cpx<T>{(T)0, (T)0};
+        if (k == 0 || 2 * k == NS_) { A.y = (T)0; B.y = (T)0; }
+        v[j] = r < NH ? cpx<T>{A.x - B.y, A.y + B.x} : cpx<T>{A.x + B.y, B.x - A.y};
+      }
+    }
+    stage_twiddles<T, TWN, TPB>(const_cast<cpx<T>*>(twl), a.tw, t);
+#pragma unroll
+    for (int j = 0; j < RP; ++j) bufA[(r0 + j * RS) * WS + w] = v[j];
+  } else if constexpr (MODE == M_C2C || MODE == M_MULT) {
     for (int r = r0; r < n; r += rstep) {
       cpx<T> v = {(T)0, (T)0};
       if (va) v = ldc<T>(a.in, ia + (int64_t)r * a.Sin);
@@ -329,7 +420,9 @@ __global__ void __launch_bounds__(256) sfft_kernel(FftArgs<T> a) {
     res = srun<T, N, WS, +1>(bufA, bufB, a, twl, W);
   } else {
     res = srun<T, N, WS, -1>(bufA, bufB, a, twl, W);
-    for (int r = r0; r < n; r += rstep) {
+#pragma unroll
+    for (int j = 0; j < rp; ++j) {
+      const int r = r0 + j * rstep;
       const cpx<T> v = res[r * W + w];
       cpx<T> o;
       if constexpr (MODE == M_DERIV) {
@@ -349,18 +442,29 @@ __global__ void __launch_bounds__(256) sfft_kernel(FftArgs<T> a) {
 
   // ---------------- store
   if constexpr (MODE == M_C2C || MODE == M_MULT) {
-    if (va)
-      for (int r = r0; r < n; r += rstep) stc<T>(a.out, oa + (int64_t)r * a.Sout, res[r * W + w]);
+    if (va) {
+#pragma unroll
+      for (int j = 0; j < rp; ++j) {
+        const int r = r0 + j * rstep;
+        stc<T>(a.out, oa + (int64_t)r * a.Sout, res[r * W + w]);
+      }
+    }
   } else if constexpr (MODE == M_R2C) {
     const T h = (T)0.5;
-    for (int k = r0; k < nh; k += rstep) {
+    const int rph = STATIC ? (NS_ / 2 + 1 + TPB / WS - 1) / (TPB / WS) : (nh + rstep - 1) / rstep;
+#pragma unroll
+    for (int j = 0; j < rph; ++j) {
+      const int k = r0 + j * rstep;
+      if (k >= nh) break;
       const cpx<T> z1 = res[k * W + w];
       const cpx<T> z2 = res[(k == 0 ? 0 : n - k) * W + w];
       if (va) stc<T>(a.out, oa + (int64_t)k * a.Sout, cpx<T>{h * (z1.x + z2.x), h * (z1.y - z2.y)});
       if (vb) stc<T>(a.out, ob + (int64_t)k * a.Sout, cpx<T>{h * (z1.y + z2.y), h * (z2.x - z1.x)});
     }
   } else {
-    for (int r = r0; r < n; r += rstep) {
+#pragma unroll
+    for (int j = 0; j < rp; ++j) {
+      const int r = r0 + j * rstep;
       const cpx<T> z = res[r * W + w];
       if constexpr (MODE == M_DERIV) {
         if (a.accumulate) {

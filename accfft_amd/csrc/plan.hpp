@@ -10,6 +10,7 @@
 #include <stdint.h>
 
 #include <cmath>
+#include <string>
 #include <vector>
 
 #include "../../include/accfft_amd.h"
@@ -35,9 +36,50 @@ struct accfft_amd_plan {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_ms = 0.f;
   int device = 0;
+  // per-launch event profiling (accfft_amd_plan_set_profiling)
+  struct ProfKind {
+    std::string name;
+    double total_ms = 0;
+    int64_t launches = 0;
+    double bytes = 0;  // algorithmic bytes of the last launch
+  };
+  struct ProfPending {
+    int kind;
+    hipEvent_t a, b;
+  };
+  bool prof = false;
+  std::vector<ProfKind> prof_kinds;
+  std::vector<ProfPending> prof_pending;
+  std::vector<hipEvent_t> prof_pool;
 };
 
 namespace afft {
+
+inline hipEvent_t prof_event(accfft_amd_plan* p) {
+  if (!p->prof_pool.empty()) {
+    hipEvent_t e = p->prof_pool.back();
+    p->prof_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+// Folds finished launch timings into the per-kind totals.
+inline int prof_collect(accfft_amd_plan* p) {
+  for (auto& q : p->prof_pending) {
+    if (hipEventSynchronize(q.b) != hipSuccess) return ACCFFT_AMD_ERR_HIP;
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, q.a, q.b);
+    p->prof_kinds[q.kind].total_ms += ms;
+    p->prof_kinds[q.kind].launches += 1;
+    p->prof_pool.push_back(q.a);
+    p->prof_pool.push_back(q.b);
+  }
+  p->prof_pending.clear();
+  return ACCFFT_AMD_OK;
+}
 
 // exp(-2 pi i m / n) in long double, rounded once to T.
 template <typename T>

@@ -183,6 +183,17 @@ int accfft_amd_plan_sync(accfft_amd_plan* plan);
  * timer, for benchmarking: device milliseconds of [first, last] launch. */
 int accfft_amd_last_elapsed_ms(const accfft_amd_plan* plan, float* ms);
 
+/* Per-launch profiling (no counterpart in the reference, whose only
+ * instrumentation is the timer[] array, doc/timing.md).  When enabled, every
+ * kernel launch of the plan is bracketed by HIP events on the plan's stream;
+ * totals are grouped by kernel kind, named "<z|s>fft_<mode>_n<N>_<axis>_<fp>".
+ * accfft_amd_profile_get synchronises on the recorded events. */
+int accfft_amd_plan_set_profiling(accfft_amd_plan* plan, int enable);
+int accfft_amd_profile_reset(accfft_amd_plan* plan);
+int accfft_amd_profile_count(accfft_amd_plan* plan, int* nkinds);
+int accfft_amd_profile_get(accfft_amd_plan* plan, int idx, char* name, int name_len,
+                           double* total_ms, int64_t* launches, double* bytes_per_launch);
+
 #ifdef __cplusplus
 }
 #endif
