@@ -49,6 +49,24 @@ def _compile(src: str) -> str:
     return obj
 
 
+ROOT = os.path.dirname(HERE)
+DROPIN_SRC = os.path.join(ROOT, "tests", "cpp", "dropin_steps.cpp")
+DROPIN_BIN = os.path.join(ROOT, "tests", "cpp", "dropin_steps")
+
+
+def build_dropin_example() -> str:
+    """Builds tests/cpp/dropin_steps, a program written against the
+    reference's C++ API, from the drop-in headers in include/."""
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-std=c++17", "-O2", "-I", os.path.join(ROOT, "include"),
+           DROPIN_SRC, "-L", HERE, "-laccfft_amd", "-Wl,-rpath,$ORIGIN/../../accfft_amd",
+           "-o", DROPIN_BIN + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("drop-in example failed to build:\n" + r.stderr)
+    os.replace(DROPIN_BIN + ".tmp", DROPIN_BIN)
+    return DROPIN_BIN
+
+
 def build(verbose: bool = True) -> str:
     os.makedirs(OBJ, exist_ok=True)
     jobs = min(len(SOURCES), int(os.environ.get("MAX_JOBS", "4")) or 1)
@@ -60,8 +78,9 @@ def build(verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr)
     os.replace(tmp, LIB)
+    build_dropin_example()
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {DROPIN_BIN}")
     return LIB
 
 

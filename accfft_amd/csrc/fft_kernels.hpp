@@ -80,6 +80,15 @@ __device__ __forceinline__ void str(void* p, int64_t i, T v) {
   reinterpret_cast<T*>(p)[i] = v;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X; this
+// maps block b to a tile so that each XCD walks one contiguous range of
+// tiles, and neighbouring tiles (which may share cache lines when rows are
+// not 128-byte aligned) meet in the same L2.  A bijection on [0, nb).
+__device__ __forceinline__ int64_t xcd_tile(unsigned b, unsigned nb) {
+  const unsigned x = b & 7u, i = b >> 3, q = nb >> 3, r = nb & 7u;
+  return (int64_t)x * q + (x < r ? x : r) + i;
+}
+
 // Copies the static engine's twiddle table into LDS with an unrolled loop
 // (issued after the caller's data loads so that all loads overlap).
 template <typename T, int TWN, int TPB>
@@ -174,6 +183,36 @@ __global__ void __launch_bounds__(N > 0 ? ZCfg<(N > 0 ? N : 16)>::TPB : 256)
     constexpr int LPW = ZCfg<NS_>::LPW;
     constexpr int EP = LPW * NS_ / TPB;  // elements per thread
     constexpr int NH = NS_ / 2 + 1;
+    if constexpr (MODE == M_C2R) {
+      // Z = A + iB on [0, NH) and conj(A) + i conj(B) on (NH, N): each
+      // half-spectrum element is read once, coalesced along k
+      constexpr int NHT = LPW * NH;
+      constexpr int EH = (NHT + TPB - 1) / TPB;
+      cpx<T> A[EH], B[EH];
+#pragma unroll
+      for (int j = 0; j < EH; ++j) {
+        const int q = t + j * TPB;
+        A[j] = B[j] = cpx<T>{(T)0, (T)0};
+        if (q < NHT) {
+          const int p = q / NH, k = q % NH;
+          const int64_t ga = 2 * (l0 + p), gb = ga + 1;
+          if (ga < nl) A[j] = ldc<T>(a.in, ga * a.idist + k);
+          if (gb < nl) B[j] = ldc<T>(a.in, gb * a.idist + k);
+        }
+      }
+      stage_twiddles<T, TWN, TPB>(const_cast<cpx<T>*>(twl), a.tw, t);
+#pragma unroll
+      for (int j = 0; j < EH; ++j) {
+        const int q = t + j * TPB;
+        if (q < NHT) {
+          const int p = q / NH, k = q % NH;
+          cpx<T> x = A[j], y = B[j];
+          if (k == 0 || 2 * k == NS_) { x.y = (T)0; y.y = (T)0; }
+          bufA[p * NS_ + k] = cpx<T>{x.x - y.y, x.y + y.x};
+          if (k != 0 && 2 * k != NS_) bufA[p * NS_ + NS_ - k] = cpx<T>{x.x + y.y, y.x - x.y};
+        }
+      }
+    } else {
     cpx<T> v[EP];
 #pragma unroll
     for (int j = 0; j < EP; ++j) {
@@ -186,18 +225,12 @@ __global__ void __launch_bounds__(N > 0 ? ZCfg<(N > 0 ? N : 16)>::TPB : 256)
         const int64_t ra = 2 * (l0 + p), rb = ra + 1;
         v[j].x = ra < nl ? ldr<T>(a.in, ra * a.idist + i) : (T)0;
         v[j].y = rb < nl ? ldr<T>(a.in, rb * a.idist + i) : (T)0;
-      } else {  // M_C2R: Z = A + iB from the two half spectra (Hermitian extension)
-        const int64_t ga = 2 * (l0 + p), gb = ga + 1;
-        const int k = i < NH ? i : NS_ - i;
-        cpx<T> A = ga < nl ? ldc<T>(a.in, ga * a.idist + k) : cpx<T>{(T)0, (T)0};
-        cpx<T> B = gb < nl ? ldc<T>(a.in, gb * a.idist + k) : cpx<T>{(T)0, (T)0};
-        if (k == 0 || 2 * k == NS_) { A.y = (T)0; B.y = (T)0; }
-        v[j] = i < NH ? cpx<T>{A.x - B.y, A.y + B.x} : cpx<T>{A.x + B.y, B.x - A.y};
       }
     }
     stage_twiddles<T, TWN, TPB>(const_cast<cpx<T>*>(twl), a.tw, t);
 #pragma unroll
     for (int j = 0; j < EP; ++j) bufA[t + j * TPB] = v[j];
+    }
   } else if constexpr (MODE == M_C2C || MODE == M_MULT) {
     for (int e = t; e < lpw * n; e += TPB) {
       const int p = e / n, i = e - p * n;
@@ -344,11 +377,12 @@ __global__ void __launch_bounds__(256) sfft_kernel(FftArgs<T> a) {
   const int64_t G = a.nlines;
   constexpr bool PAIR = (MODE == M_R2C || MODE == M_C2R || MODE == M_DERIV);
   int64_t ga, gb;  // columns (real columns for pair modes)
+  const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
   if constexpr (PAIR) {
-    ga = 2 * ((int64_t)blockIdx.x * W + w);
+    ga = 2 * (tile * W + w);
     gb = ga + 1;
   } else {
-    ga = (int64_t)blockIdx.x * W + w;
+    ga = tile * W + w;
     gb = ga;
   }
   const bool va = ga < G, vb = gb < G;

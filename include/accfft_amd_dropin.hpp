@@ -73,7 +73,7 @@ inline void fill_layouts(P* p, bool r2c) {
   p->osize_yi[0] = n0; p->osize_yi[1] = n1 / 2 + 1; p->osize_yi[2] = n2;
   p->osize_x[0] = n0; p->osize_x[1] = n1 * n2; p->osize_x[2] = 1;
   p->osize_xi[0] = n0 / 2 + 1; p->osize_xi[1] = n1 * n2; p->osize_xi[2] = 1;
-  p->ostart_x[0] = 0; p->ostart_x[1] = -1 << 8; p->ostart_x[2] = -1 << 8;
+  p->ostart_x[0] = 0; p->ostart_x[1] = -256; p->ostart_x[2] = -256;  // -1<<8 upstream
 }
 
 template <class P>
