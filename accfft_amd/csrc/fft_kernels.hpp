@@ -42,6 +42,8 @@ struct FftArgs {
   int64_t Ch;                        // M_MULT: column g -> (g/Ch, g%Ch)
   int maxis;                         // M_MULT: axis of the rows (0: x, 1: y)
   int accumulate;                    // M_DERIV: out += result
+  int xgroup;                        // s: tiles per XCD chunk (see xcd_tile)
+  int reverse;                       // process blocks last-to-first
 };
 
 // -------------------------------------------------------------- configs
@@ -71,22 +73,48 @@ template <typename T>
 __device__ __forceinline__ T ldr(const void* p, int64_t i) {
   return reinterpret_cast<const T*>(p)[i];
 }
+// Non-temporal result stores (AFFT_NT_STORES=1) were measured slower on
+// MI355X: the y pass of the r2c layout writes rows that straddle 128-byte
+// lines, and partial-line non-temporal writes cannot merge in L2.
+#ifndef AFFT_NT_STORES
+#define AFFT_NT_STORES 0
+#endif
 template <typename T>
 __device__ __forceinline__ void stc(void* p, int64_t i, cpx<T> v) {
+#if AFFT_NT_STORES
+  T* q = reinterpret_cast<T*>(p) + 2 * i;
+  __builtin_nontemporal_store(v.x, q);
+  __builtin_nontemporal_store(v.y, q + 1);
+#else
   reinterpret_cast<cpx<T>*>(p)[i] = v;
+#endif
 }
 template <typename T>
 __device__ __forceinline__ void str(void* p, int64_t i, T v) {
+#if AFFT_NT_STORES
+  __builtin_nontemporal_store(v, reinterpret_cast<T*>(p) + i);
+#else
   reinterpret_cast<T*>(p)[i] = v;
+#endif
 }
 
-// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X; this
-// maps block b to a tile so that each XCD walks one contiguous range of
-// tiles, and neighbouring tiles (which may share cache lines when rows are
-// not 128-byte aligned) meet in the same L2.  A bijection on [0, nb).
-__device__ __forceinline__ int64_t xcd_tile(unsigned b, unsigned nb) {
-  const unsigned x = b & 7u, i = b >> 3, q = nb >> 3, r = nb & 7u;
-  return (int64_t)x * q + (x < r ? x : r) + i;
+// Workgroups are dispatched round-robin over the 8 XCDs of an MI355X.  This
+// maps block b to a tile so that each XCD takes chunks of g consecutive
+// tiles: neighbouring tiles (which share cache lines when rows are not
+// 128-byte aligned) meet in one L2, while the tiles in flight still cover a
+// contiguous address range (all HBM channels busy).  g = 1 is the identity.
+// A bijection on [0, nb): blocks past the last whole group of 8*g keep
+// their index.
+// g = 0 gives each XCD one contiguous range.
+__device__ __forceinline__ int64_t xcd_tile(unsigned b, unsigned nb, unsigned g) {
+  if (g == 0u) {
+    const unsigned x = b & 7u, i = b >> 3, q = nb >> 3, r = nb & 7u;
+    return (int64_t)x * q + (x < r ? x : r) + i;
+  }
+  const unsigned full = (nb / (8u * g)) * 8u * g;
+  if (g == 1u || b >= full) return b;
+  const unsigned x = b & 7u, i = b >> 3;
+  return (int64_t)(i / g) * 8u * g + (int64_t)x * g + (i % g);
 }
 
 // Copies the static engine's twiddle table into LDS with an unrolled loop
@@ -173,7 +201,10 @@ __global__ void __launch_bounds__(N > 0 ? ZCfg<(N > 0 ? N : 16)>::TPB : 256)
     twl = a.tw;
   }
 
-  const int64_t l0 = (int64_t)blockIdx.x * lpw;  // first complex line / pair
+  // reverse: walk the lines last-to-first, so a pass can begin with the data
+  // its predecessor wrote last (still in the memory-side cache)
+  const int64_t blk = a.reverse ? (int64_t)gridDim.x - 1 - blockIdx.x : (int64_t)blockIdx.x;
+  const int64_t l0 = blk * lpw;  // first complex line / pair
   const int64_t nl = a.nlines;
 
   // ---------------- load
@@ -377,7 +408,8 @@ __global__ void __launch_bounds__(256) sfft_kernel(FftArgs<T> a) {
   const int64_t G = a.nlines;
   constexpr bool PAIR = (MODE == M_R2C || MODE == M_C2R || MODE == M_DERIV);
   int64_t ga, gb;  // columns (real columns for pair modes)
-  const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+  int64_t tile = xcd_tile(blockIdx.x, gridDim.x, (unsigned)a.xgroup);
+  if (a.reverse) tile = (int64_t)gridDim.x - 1 - tile;
   if constexpr (PAIR) {
     ga = 2 * (tile * W + w);
     gb = ga + 1;

@@ -72,6 +72,21 @@ def run_gpu(n, fp, steps, warmup, profile=True):
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / steps
 
+    # practical HBM ceiling on this box: a device-to-device copy of the
+    # same footprint (read + write), for context next to the FFT passes
+    src = spec.view(torch.float32) if fp == af.F32 else spec.view(torch.float64)
+    dst = torch.empty_like(src)
+    for _ in range(2):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(5):
+        dst.copy_(src)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    copy_gbps = 2 * src.numel() * src.element_size() / (e0.elapsed_time(e1) / 5 * 1e-3) / 1e9
+    del dst
+
     prof = {}
     if profile:
         plan.set_profiling(True)
@@ -82,7 +97,7 @@ def run_gpu(n, fp, steps, warmup, profile=True):
         prof = plan.profile()
         plan.set_profiling(False)
     plan.destroy()
-    return ms, err, prof
+    return ms, err, prof, copy_gbps
 
 
 def cpu_baseline(n, budget_s=30.0):
@@ -137,8 +152,8 @@ def main():
     import accfft_amd as af
 
     n = (args.n, args.n, args.n)
-    ms, err, prof = run_gpu(n, af.F32, args.steps, args.warmup)
-    log(f"fp32 {n} round trip {ms:.4f} ms, rel err {err:.3e}")
+    ms, err, prof, copy_gbps = run_gpu(n, af.F32, args.steps, args.warmup)
+    log(f"fp32 {n} round trip {ms:.4f} ms, rel err {err:.3e}; device copy {copy_gbps:.0f} GB/s")
     for k, (tot, cnt, b) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
         log(f"  {k:40s} avg {tot / max(cnt, 1) * 1e3:8.1f} us  {b / (tot / max(cnt, 1)) / 1e6:8.1f} GB/s"
             f"  x{cnt}")
@@ -152,7 +167,7 @@ def main():
 
     extra = {}
     if not args.no_fp64:
-        ms64, err64, _ = run_gpu(n, af.F64, max(5, args.steps // 2), 2, profile=False)
+        ms64, err64, _, _ = run_gpu(n, af.F64, max(5, args.steps // 2), 2, profile=False)
         alg64, _ = alg_bytes_roundtrip(n, 8)
         extra = {"fp64_ms_per_step": round(ms64, 4), "fp64_rel_err": err64,
                  "fp64_pass_model_frac": round(alg64 / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
@@ -178,6 +193,7 @@ def main():
             "pass_model_bytes": alg,
             "pass_model_frac_of_hbm_peak": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "compulsory_frac_of_hbm_peak": round(comp / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "device_copy_GBps": round(copy_gbps, 1),
             "kernels": {k: {"avg_us": round(v[0] / max(v[1], 1) * 1e3, 2),
                             "alg_GBps": round(v[2] / (v[0] / max(v[1], 1) * 1e-3) / 1e9, 1)}
                         for k, v in prof.items()},
