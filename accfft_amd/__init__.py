@@ -41,6 +41,12 @@ class AccfftError(RuntimeError):
     pass
 
 
+# int (*)(const void* send, void* recv, int64_t block_bytes, int nprocs,
+#         int direction, void* stream, void* user)
+EXCHANGE_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                               ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p)
+
+
 def lib() -> ctypes.CDLL:
     """Loads libaccfft_amd.so; raises if it has not been built."""
     global _lib
@@ -58,6 +64,9 @@ def lib() -> ctypes.CDLL:
         getattr(L, nm).argtypes = [_i3] * 5 + [ctypes.c_int] * 5 + [i64p]
     for nm in ("accfft_amd_plan_dft_3d_r2c", "accfft_amd_plan_dft_3d_c2c"):
         getattr(L, nm).argtypes = [_i3, _vp, _vp, ctypes.c_int, ctypes.c_uint, ctypes.POINTER(_vp)]
+    L.accfft_amd_plan_dft_3d_r2c_slab.argtypes = [_i3, ctypes.c_int, _vp, _vp, ctypes.c_int,
+                                                  ctypes.c_uint, ctypes.POINTER(_vp)]
+    L.accfft_amd_plan_set_exchange.argtypes = [_vp, EXCHANGE_FN, _vp]
     L.accfft_amd_destroy_plan.argtypes = [_vp]
     L.accfft_amd_plan_set_stream.argtypes = [_vp, _vp]
     L.accfft_amd_plan_sync.argtypes = [_vp]
@@ -84,6 +93,12 @@ def lib() -> ctypes.CDLL:
                                    _vp, _vp, ctypes.c_int, _vp]
     L.accfft_amd_local_transpose_inplace.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64,
                                                      _vp, ctypes.c_int, _vp]
+    L.accfft_amd_adjoint_r2c.argtypes = [_vp, _vp, _vp, dp]
+    L.accfft_amd_adjoint_c2r.argtypes = [_vp, _vp, _vp, dp]
+    L.accfft_amd_adjoint_grad.argtypes = [_vp, _vp, _vp, _vp, _vp, dp]
+    L.accfft_amd_adjoint_divergence.argtypes = [_vp, _vp, _vp, _vp, _vp, dp]
+    L.accfft_amd_plan_release_workspace.argtypes = [_vp]
+    L.accfft_amd_plan_workspace_bytes.argtypes = [_vp, i64p]
     L.accfft_amd_last_elapsed_ms.argtypes = [_vp, ctypes.POINTER(ctypes.c_float)]
     L.accfft_amd_plan_set_profiling.argtypes = [_vp, ctypes.c_int]
     L.accfft_amd_profile_reset.argtypes = [_vp]
@@ -151,14 +166,24 @@ class Plan:
     same buffer, which includes passing neither."""
 
     def __init__(self, kind: str, n: Sequence[int], data=None, data_out=None,
-                 precision: int = F64, flags: int = ACCFFT_MEASURE):
+                 precision: int = F64, flags: int = ACCFFT_MEASURE, nprocs: int = 1):
+        """``kind`` is "r2c", "c2c" or "r2c_slab" (one rank's share of an r2c
+        on an ``nprocs`` x 1 grid, ``n`` global; see include/accfft_amd.h)."""
         self.kind = kind
         self.n = tuple(int(v) for v in n)
         self.precision = precision
+        self.nprocs = nprocs
         h = _vp()
-        fn = lib().accfft_amd_plan_dft_3d_r2c if kind == "r2c" else lib().accfft_amd_plan_dft_3d_c2c
-        _check(fn(_i3(*self.n), _ptr(data), _ptr(data_out), precision, flags, ctypes.byref(h)),
-               f"plan_dft_3d_{kind}")
+        if kind == "r2c_slab":
+            _check(lib().accfft_amd_plan_dft_3d_r2c_slab(_i3(*self.n), nprocs, _ptr(data),
+                                                         _ptr(data_out), precision, flags,
+                                                         ctypes.byref(h)), "plan_dft_3d_r2c_slab")
+        else:
+            fn = (lib().accfft_amd_plan_dft_3d_r2c if kind == "r2c"
+                  else lib().accfft_amd_plan_dft_3d_c2c)
+            _check(fn(_i3(*self.n), _ptr(data), _ptr(data_out), precision, flags,
+                      ctypes.byref(h)), f"plan_dft_3d_{kind}")
+        self._exchange = None
         self._h = h
         isz, ist, osz, ost = _i3(), _i3(), _i3(), _i3()
         amax = ctypes.c_int64()
@@ -265,6 +290,56 @@ class Plan:
         _check(lib().accfft_amd_divergence(self._bind(), _ptr(out), _ptr(Ax), _ptr(Ay), _ptr(Az),
                                            tp, int(slow)), "divergence")
         self._untimer(timer, arr)
+
+    def set_exchange(self, fn):
+        """Exchange hook of a slab plan: ``fn(send, recv, block_bytes, nprocs,
+        direction, stream)`` (ints) returning a status; None = identity."""
+        if fn is None:
+            cb = EXCHANGE_FN()
+        else:
+            def tramp(send, recv, nbytes, nprocs, direction, stream, user):
+                try:
+                    return int(fn(send, recv, nbytes, nprocs, direction, stream))
+                except Exception:  # a Python error must not unwind through C
+                    return 5
+            cb = EXCHANGE_FN(tramp)
+        _check(lib().accfft_amd_plan_set_exchange(self._h, cb, None), "set_exchange")
+        self._exchange = cb  # keep the thunk alive while the plan holds it
+
+    # adjoints of the real-linear maps above (no counterpart in the reference)
+    def adjoint_r2c(self, G, x, timer=None):
+        """x = r2c^T(G): the gradient of <G, r2c(x)> with respect to x."""
+        arr, tp = self._timer(timer)
+        _check(lib().accfft_amd_adjoint_r2c(self._bind(), _ptr(G), _ptr(x), tp), "adjoint_r2c")
+        self._untimer(timer, arr)
+
+    def adjoint_c2r(self, y, G, timer=None):
+        """G = c2r^T(y); ``y`` is not overwritten."""
+        arr, tp = self._timer(timer)
+        _check(lib().accfft_amd_adjoint_c2r(self._bind(), _ptr(y), _ptr(G), tp), "adjoint_c2r")
+        self._untimer(timer, arr)
+
+    def adjoint_grad(self, gx, gy, gz, out, timer=None):
+        """out = grad^T(gx, gy, gz) = -div(gx, gy, gz); None components are skipped."""
+        arr, tp = self._timer(timer)
+        _check(lib().accfft_amd_adjoint_grad(self._bind(), _ptr(out), _ptr(gx), _ptr(gy),
+                                             _ptr(gz), tp), "adjoint_grad")
+        self._untimer(timer, arr)
+
+    def adjoint_divergence(self, v, ox=None, oy=None, oz=None, timer=None):
+        """(ox, oy, oz) = div^T(v) = -grad(v); None outputs are skipped."""
+        arr, tp = self._timer(timer)
+        _check(lib().accfft_amd_adjoint_divergence(self._bind(), _ptr(ox), _ptr(oy), _ptr(oz),
+                                                   _ptr(v), tp), "adjoint_divergence")
+        self._untimer(timer, arr)
+
+    def release_workspace(self):
+        _check(lib().accfft_amd_plan_release_workspace(self._bind()), "release_workspace")
+
+    def workspace_bytes(self) -> int:
+        v = ctypes.c_int64()
+        _check(lib().accfft_amd_plan_workspace_bytes(self._h, ctypes.byref(v)), "workspace_bytes")
+        return v.value
 
     def sync(self):
         _check(lib().accfft_amd_plan_sync(self._bind()), "sync")

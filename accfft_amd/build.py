@@ -33,7 +33,7 @@ FLAGS = [
     # no packed FP32 instructions anywhere in the device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
-SOURCES = ["fft_f32.hip", "fft_f64.hip", "transpose.hip", "capi.hip"]
+SOURCES = ["fft_f32.hip", "fft_f64.hip", "fft_split.hip", "transpose.hip", "capi.hip"]
 
 
 def _compile(src: str) -> str:

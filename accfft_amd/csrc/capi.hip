@@ -137,6 +137,13 @@ int make_plan(int kind, const int n[3], void* data, void* data_out, int fp, unsi
   p->data = data;
   p->data_out = data_out;
   p->inplace = (data == data_out);
+  if (flags & ACCFFT_AMD_SPLIT_FP16) p->split = afft::SPLIT_FP16;
+  else if (flags & ACCFFT_AMD_SPLIT_BF16X3) p->split = afft::SPLIT_BF16X3;
+  else if (flags & ACCFFT_AMD_SPLIT_BF16) p->split = afft::SPLIT_BF16;
+  if (p->split && fp != ACCFFT_AMD_F32) {
+    delete p;
+    return ACCFFT_AMD_ERR_INVALID;
+  }
   const int elem = elem_of(fp);
   p->alloc_max = kind == 1 ? size_r2c(n, p->isize, p->istart, p->osize, p->ostart, 1, 1, 0, 0, elem)
                            : size_c2c(n, p->isize, p->istart, p->osize, p->ostart, 1, 1, 0, 0, elem);
@@ -227,6 +234,33 @@ int accfft_amd_plan_dft_3d_c2c(const int n[3], void* data, void* data_out, int f
   return make_plan(2, n, data, data_out, fp, flags, plan);
 }
 
+int accfft_amd_plan_dft_3d_r2c_slab(const int n[3], int nprocs, void* data, void* data_out,
+                                    int fp, unsigned flags, accfft_amd_plan** plan) {
+  if (!plan || !n) return ACCFFT_AMD_ERR_NULL;
+  *plan = nullptr;
+  if (nprocs < 1 || n[0] <= 0 || n[1] <= 0 || n[2] <= 0) return ACCFFT_AMD_ERR_INVALID;
+  if (n[0] % nprocs || n[1] % nprocs) return ACCFFT_AMD_ERR_UNSUPPORTED;
+  const int n1 = n[1] / nprocs;
+  if (nprocs > 1 && (n1 & (n1 - 1))) return ACCFFT_AMD_ERR_UNSUPPORTED;
+  if (data == data_out) return ACCFFT_AMD_ERR_UNSUPPORTED;  // out of place only
+  accfft_amd_plan* p = nullptr;
+  int st = make_plan(3, n, data, data_out, fp, flags, &p);
+  if (st) return st;
+  p->nprocs = nprocs;
+  p->alloc_max = size_r2c(n, p->isize, p->istart, p->osize, p->ostart, nprocs, 1, 0, 0,
+                          elem_of(fp));
+  *plan = p;
+  return ACCFFT_AMD_OK;
+}
+
+int accfft_amd_plan_set_exchange(accfft_amd_plan* p, accfft_amd_exchange_fn fn, void* user) {
+  if (!p) return ACCFFT_AMD_ERR_NULL;
+  if (p->kind != 3) return ACCFFT_AMD_ERR_NOT_BAKED;
+  p->exchange = fn;
+  p->exchange_user = user;
+  return ACCFFT_AMD_OK;
+}
+
 int accfft_amd_destroy_plan(accfft_amd_plan* plan) {
   if (!plan) return ACCFFT_AMD_ERR_NULL;
   free_plan(plan);
@@ -267,11 +301,17 @@ int accfft_amd_plan_data(const accfft_amd_plan* p, void** data, void** data_out)
 int accfft_amd_execute_r2c(accfft_amd_plan* p, const void* in, void* out, double* timer,
                            unsigned xyz) {
   if (!p) return ACCFFT_AMD_ERR_NULL;
-  if (p->kind != 1) return ACCFFT_AMD_ERR_NOT_BAKED;
+  if (p->kind != 1 && p->kind != 3) return ACCFFT_AMD_ERR_NOT_BAKED;
   if (!in) in = p->data;
   if (!out) out = p->data_out;
   if (!in || !out) return ACCFFT_AMD_ERR_NULL;
   xyz &= 7u;
+  if (p->kind == 3) {
+    if (xyz != 7u) return ACCFFT_AMD_ERR_UNSUPPORTED;
+    return timed(p, timer, [&] {
+      return p->fp == ACCFFT_AMD_F32 ? afft_r2c_slab_f32(p, in, out) : afft_r2c_slab_f64(p, in, out);
+    });
+  }
   return timed(p, timer, [&] {
     return p->fp == ACCFFT_AMD_F32 ? afft_r2c_f32(p, in, out, xyz) : afft_r2c_f64(p, in, out, xyz);
   });
@@ -280,11 +320,17 @@ int accfft_amd_execute_r2c(accfft_amd_plan* p, const void* in, void* out, double
 int accfft_amd_execute_c2r(accfft_amd_plan* p, void* in, void* out, double* timer,
                            unsigned xyz) {
   if (!p) return ACCFFT_AMD_ERR_NULL;
-  if (p->kind != 1) return ACCFFT_AMD_ERR_NOT_BAKED;
+  if (p->kind != 1 && p->kind != 3) return ACCFFT_AMD_ERR_NOT_BAKED;
   if (!in) in = p->data_out;
   if (!out) out = p->data;
   if (!in || !out) return ACCFFT_AMD_ERR_NULL;
   xyz &= 7u;
+  if (p->kind == 3) {
+    if (xyz != 7u) return ACCFFT_AMD_ERR_UNSUPPORTED;
+    return timed(p, timer, [&] {
+      return p->fp == ACCFFT_AMD_F32 ? afft_c2r_slab_f32(p, in, out) : afft_c2r_slab_f64(p, in, out);
+    });
+  }
   return timed(p, timer, [&] {
     return p->fp == ACCFFT_AMD_F32 ? afft_c2r_f32(p, in, out, xyz) : afft_c2r_f64(p, in, out, xyz);
   });
@@ -405,6 +451,59 @@ int accfft_amd_axpy(int fp, int64_t n, double alpha, const void* x, void* y, voi
 int accfft_amd_plan_sync(accfft_amd_plan* p) {
   if (!p) return ACCFFT_AMD_ERR_NULL;
   return hipStreamSynchronize(p->stream) == hipSuccess ? ACCFFT_AMD_OK : ACCFFT_AMD_ERR_HIP;
+}
+
+int accfft_amd_adjoint_r2c(accfft_amd_plan* p, const void* G, void* x, double* timer) {
+  if (!p || !G || !x) return ACCFFT_AMD_ERR_NULL;
+  if (p->kind != 1) return ACCFFT_AMD_ERR_NOT_BAKED;
+  return timed(p, timer, [&] {
+    return p->fp == ACCFFT_AMD_F32 ? afft_adj_r2c_f32(p, G, x) : afft_adj_r2c_f64(p, G, x);
+  });
+}
+
+int accfft_amd_adjoint_c2r(accfft_amd_plan* p, const void* y, void* G, double* timer) {
+  if (!p || !y || !G) return ACCFFT_AMD_ERR_NULL;
+  if (p->kind != 1) return ACCFFT_AMD_ERR_NOT_BAKED;
+  return timed(p, timer, [&] {
+    return p->fp == ACCFFT_AMD_F32 ? afft_adj_c2r_f32(p, y, G) : afft_adj_c2r_f64(p, y, G);
+  });
+}
+
+int accfft_amd_adjoint_grad(accfft_amd_plan* p, void* out, const void* gx, const void* gy,
+                            const void* gz, double* timer) {
+  if (!p || !out) return ACCFFT_AMD_ERR_NULL;
+  if (p->kind != 1) return ACCFFT_AMD_ERR_NOT_BAKED;
+  return timed(p, timer, [&] {
+    return p->fp == ACCFFT_AMD_F32 ? afft_adj_grad_f32(p, out, gx, gy, gz)
+                                   : afft_adj_grad_f64(p, out, gx, gy, gz);
+  });
+}
+
+int accfft_amd_adjoint_divergence(accfft_amd_plan* p, void* ox, void* oy, void* oz,
+                                  const void* v, double* timer) {
+  if (!p || !v) return ACCFFT_AMD_ERR_NULL;
+  if (p->kind != 1) return ACCFFT_AMD_ERR_NOT_BAKED;
+  return timed(p, timer, [&] {
+    return p->fp == ACCFFT_AMD_F32 ? afft_adj_div_f32(p, ox, oy, oz, v)
+                                   : afft_adj_div_f64(p, ox, oy, oz, v);
+  });
+}
+
+int accfft_amd_plan_release_workspace(accfft_amd_plan* p) {
+  if (!p) return ACCFFT_AMD_ERR_NULL;
+  if (hipStreamSynchronize(p->stream) != hipSuccess) return ACCFFT_AMD_ERR_HIP;
+  for (int i = 0; i < 2; ++i) {
+    if (p->ws[i]) (void)hipFree(p->ws[i]);
+    p->ws[i] = nullptr;
+    p->ws_bytes[i] = 0;
+  }
+  return ACCFFT_AMD_OK;
+}
+
+int accfft_amd_plan_workspace_bytes(const accfft_amd_plan* p, int64_t* bytes) {
+  if (!p || !bytes) return ACCFFT_AMD_ERR_NULL;
+  *bytes = (int64_t)(p->ws_bytes[0] + p->ws_bytes[1]);
+  return ACCFFT_AMD_OK;
 }
 
 int accfft_amd_plan_set_profiling(accfft_amd_plan* p, int enable) {

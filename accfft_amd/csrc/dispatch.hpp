@@ -19,7 +19,14 @@ struct accfft_amd_plan;
                      const void* Az, int slow);                                             \
   int afft_mult_##SUF(int kind, void* wA, const void* A, const int* n, const int* osize,    \
                       const int* ostart, double scale, void* stream);                       \
-  int afft_axpy_##SUF(int64_t n, double alpha, const void* x, void* y, void* stream);
+  int afft_axpy_##SUF(int64_t n, double alpha, const void* x, void* y, void* stream);       \
+  int afft_r2c_slab_##SUF(accfft_amd_plan* p, const void* in, void* out);                   \
+  int afft_c2r_slab_##SUF(accfft_amd_plan* p, void* in, void* out);                         \
+  int afft_adj_r2c_##SUF(accfft_amd_plan* p, const void* G, void* x);                       \
+  int afft_adj_c2r_##SUF(accfft_amd_plan* p, const void* y, void* G);                       \
+  int afft_adj_grad_##SUF(accfft_amd_plan* p, void* out, const void* gx, const void* gy,    \
+                          const void* gz);                                                  \
+  int afft_adj_div_##SUF(accfft_amd_plan* p, void* ox, void* oy, void* oz, const void* v);
 
 AFFT_DECLARE(f32)
 AFFT_DECLARE(f64)

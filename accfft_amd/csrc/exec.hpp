@@ -91,7 +91,14 @@ inline int run_pass(accfft_amd_plan* p, const char* tag, bool strided, int mode,
     const bool real_in = (mode == M_R2C || mode == M_DERIV);
     a.xgroup = xcd_group_for(a.Sin * (int64_t)(real_in ? sizeof(T) : 2 * sizeof(T)));
   }
-  if (!p->prof) return status_of(launch_fft<T>(strided, mode, sign, a, tab, p->stream));
+  auto launch = [&]() -> hipError_t {
+    if constexpr (sizeof(T) == 4) {
+      if (p->split && tab.tw_split && split_mode(mode))
+        return launch_fft_split(strided, mode, sign, a, tab, p->split, p->stream);
+    }
+    return launch_fft<T>(strided, mode, sign, a, tab, p->stream);
+  };
+  if (!p->prof) return status_of(launch());
   const int64_t n = tab.n;
   const int64_t nh = n / 2 + 1;
   const double rs = sizeof(T), cs = 2.0 * sizeof(T);
@@ -103,8 +110,11 @@ inline int run_pass(accfft_amd_plan* p, const char* tag, bool strided, int mode,
     default: bytes = a.nlines * (2.0 * n * cs); break;
   }
   char name[96];
-  snprintf(name, sizeof(name), "%cfft_%s_n%d_%s_%s", strided ? 's' : 'z', mode_name<T>(mode, sign),
-           (int)n, tag, sizeof(T) == 4 ? "f32" : "f64");
+  const bool split = sizeof(T) == 4 && p->split && tab.tw_split && split_mode(mode);
+  static const char* sname[4] = {"", "_bf16x2", "_bf16x3", "_fp16x2"};
+  snprintf(name, sizeof(name), "%cfft_%s_n%d_%s_%s%s", strided ? 's' : 'z',
+           mode_name<T>(mode, sign), (int)n, tag, sizeof(T) == 4 ? "f32" : "f64",
+           split ? sname[p->split & 3] : "");
   int kind = -1;
   for (size_t i = 0; i < p->prof_kinds.size(); ++i)
     if (p->prof_kinds[i].name == name) kind = (int)i;
@@ -117,7 +127,7 @@ inline int run_pass(accfft_amd_plan* p, const char* tag, bool strided, int mode,
   p->prof_kinds[kind].bytes = bytes;
   hipEvent_t e0 = prof_event(p), e1 = prof_event(p);
   (void)hipEventRecord(e0, p->stream);
-  const int st = status_of(launch_fft<T>(strided, mode, sign, a, tab, p->stream));
+  const int st = status_of(launch());
   (void)hipEventRecord(e1, p->stream);
   p->prof_pending.push_back({kind, e0, e1});
   if (p->prof_pending.size() > 4096) return st ? st : prof_collect(p);
@@ -211,7 +221,8 @@ struct Ops {
     return run_pass<T>(p, "y", true, M_MULT, -1, a, tab(p, 1));
   }
   // fused per-axis derivative of a real field (operators_gpu.txx:172-278)
-  static int deriv(accfft_amd_plan* p, int axis, const void* in, void* out, int accumulate) {
+  static int deriv(accfft_amd_plan* p, int axis, const void* in, void* out, int accumulate,
+                   double sign = 1.0) {
     const int64_t N0 = p->n[0], N1 = p->n[1], N2 = p->n[2];
     FftArgs<T> a;
     bool strided = true;
@@ -223,7 +234,7 @@ struct Ops {
     } else {
       a = sargs(in, out, N1 * N2, N1 * N2, 0, 0, N1 * N2, N1 * N2);
     }
-    a.scale = 1.0 / (double)p->n[axis];
+    a.scale = sign / (double)p->n[axis];
     a.accumulate = accumulate;
     return run_pass<T>(p, axis == 0 ? "x" : axis == 1 ? "y" : "z", strided, M_DERIV, -1, a, tab(p, axis));
   }
@@ -321,6 +332,122 @@ struct Ops {
     if (axis == 1) a = sargs(in, out, N0 * N2, N2, (N1 / 2 + 1) * N2, N1 * N2, N2, N2);
     else a = sargs(in, out, N1 * N2, N1 * N2, 0, 0, N1 * N2, N1 * N2);
     return run_pass<T>(p, axis == 0 ? "x" : "y", true, M_C2R, +1, a, tab(p, axis));
+  }
+
+  // ---------------------------------------------------------- slab share
+  // One rank's share of an r2c on a P x 1 grid (src/accfft_gpu.cpp:471-539
+  // with T_plan_2 of src/transpose_gpu.cpp:3809-4021 between the y and x
+  // passes).  Input [N0/P][N1][N2] real, output [N0][N1/P][Nh]; the pack
+  // (local_transpose_col into [P][N0/P][N1/P][Nh], transpose_gpu.cpp:3889)
+  // is the y pass's store, the exchange is the plan's hook (identity when
+  // none: the packed send buffer is the receive buffer), the x pass reads
+  // the received [N0][N1/P][Nh] in place.
+  static int r2c_slab(accfft_amd_plan* p, const void* in, void* out) {
+    const int64_t N0 = p->n[0], N1 = p->n[1], N2 = p->n[2], Nh = N2 / 2 + 1;
+    const int P = p->nprocs;
+    const int64_t n0 = N0 / P, n1 = N1 / P;
+    const size_t cb = (size_t)n0 * N1 * Nh * sizeof(C);
+    int st;
+    if ((st = ensure_ws(p, 0, cb))) return st;
+    if (p->exchange && (st = ensure_ws(p, 1, cb))) return st;
+    void* zbuf = p->ws[0];
+    void* send = p->exchange ? p->ws[1] : out;
+    auto az = zargs(in, zbuf, n0 * N1, N2, Nh);
+    if ((st = run_pass<T>(p, "z", false, M_R2C, -1, az, tab(p, 2), 0))) return st;
+    auto ay = sargs(zbuf, send, n0 * Nh, Nh, N1 * Nh, n1 * Nh, Nh, Nh);
+    ay.oshift = ilog2(n1);
+    ay.Bout = n0 * n1 * Nh;
+    if (P == 1) ay.oshift = 0;
+    if ((st = run_pass<T>(p, "y", true, M_C2C, -1, ay, tab(p, 1), 1))) return st;
+    if (p->exchange &&
+        (st = p->exchange(send, out, (int64_t)(n0 * n1 * Nh * sizeof(C)), P, -1, p->stream,
+                          p->exchange_user)))
+      return st;
+    auto ax = sargs(out, out, n1 * Nh, n1 * Nh, 0, 0, n1 * Nh, n1 * Nh);
+    return run_pass<T>(p, "x", true, M_C2C, -1, ax, tab(p, 0), 2);
+  }
+  // The inverse: x pass in place on the input (which it overwrites, as the
+  // reference's c2r does), exchange, y pass reading the received blocks
+  // (the unpack local_transpose of transpose_gpu.cpp:1411 as its load), z c2r.
+  static int c2r_slab(accfft_amd_plan* p, void* in, void* out) {
+    const int64_t N0 = p->n[0], N1 = p->n[1], N2 = p->n[2], Nh = N2 / 2 + 1;
+    const int P = p->nprocs;
+    const int64_t n0 = N0 / P, n1 = N1 / P;
+    const size_t cb = (size_t)n0 * N1 * Nh * sizeof(C);
+    int st;
+    if ((st = ensure_ws(p, 0, cb))) return st;
+    if (p->exchange && (st = ensure_ws(p, 1, cb))) return st;
+    auto ax = sargs(in, in, n1 * Nh, n1 * Nh, 0, 0, n1 * Nh, n1 * Nh);
+    if ((st = run_pass<T>(p, "x", true, M_C2C, +1, ax, tab(p, 0), 3))) return st;
+    const void* recv = in;
+    if (p->exchange) {
+      if ((st = p->exchange(in, p->ws[1], (int64_t)(n0 * n1 * Nh * sizeof(C)), P, +1, p->stream,
+                            p->exchange_user)))
+        return st;
+      recv = p->ws[1];
+    }
+    void* ybuf = p->ws[0];
+    auto ay = sargs(recv, ybuf, n0 * Nh, Nh, n1 * Nh, N1 * Nh, Nh, Nh);
+    ay.ishift = ilog2(n1);
+    ay.Bin = n0 * n1 * Nh;
+    if (P == 1) ay.ishift = 0;
+    if ((st = run_pass<T>(p, "y", true, M_C2C, +1, ay, tab(p, 1), 4))) return st;
+    auto az = zargs(ybuf, out, n0 * N1, Nh, N2);
+    return run_pass<T>(p, "z", false, M_C2R, +1, az, tab(p, 2), 5);
+  }
+  static int ilog2(int64_t v) {
+    int s = 0;
+    while ((int64_t(1) << s) < v) ++s;
+    return s;
+  }
+
+  // ---------------------------------------------------------- adjoints
+  // The operators here are real-linear maps; with <a, b> = Re sum conj(a) b
+  // on spectra and the plain dot product on real fields:
+  //   r2c^T(G) = c2r(w G),  w = 1/2 on z bins other than DC / Nyquist
+  //   c2r^T(y) = (1/w) r2c(y)
+  // Neither overwrites its input.
+  static int adjoint_r2c(accfft_amd_plan* p, const void* G, void* x) {
+    const int64_t N0 = p->n[0], N1 = p->n[1], N2 = p->n[2], Nh = N2 / 2 + 1;
+    int st;
+    if ((st = ensure_ws(p, 0, (size_t)N0 * N1 * Nh * sizeof(C)))) return st;
+    void* w = p->ws[0];
+    if ((st = x_c2c(p, G, w, Nh, +1))) return st;
+    if ((st = y_c2c(p, w, w, Nh, +1))) return st;
+    auto a = zargs(w, x, N0 * N1, Nh, N2);
+    a.adjw = 1;
+    return run_pass<T>(p, "z", false, M_C2R, +1, a, tab(p, 2));
+  }
+  static int adjoint_c2r(accfft_amd_plan* p, const void* y, void* G) {
+    const int64_t N0 = p->n[0], N1 = p->n[1], N2 = p->n[2], Nh = N2 / 2 + 1;
+    int st;
+    auto a = zargs(y, G, N0 * N1, N2, Nh);
+    a.adjw = 1;
+    if ((st = run_pass<T>(p, "z", false, M_R2C, -1, a, tab(p, 2)))) return st;
+    if ((st = y_c2c(p, G, G, Nh, -1))) return st;
+    return x_c2c(p, G, G, Nh, -1);
+  }
+  // grad^T(gx, gy, gz) = -div(gx, gy, gz); div^T(v) = -grad(v): the per-axis
+  // derivative with the symmetric Nyquist filter is skew-adjoint.
+  static int adjoint_grad(accfft_amd_plan* p, void* out, const void* gx, const void* gy,
+                          const void* gz) {
+    const void* g[3] = {gx, gy, gz};
+    int st, first = 1;
+    for (int d = 0; d < 3; ++d) {
+      if (!g[d]) continue;
+      if ((st = deriv(p, d, g[d], out, first ? 0 : 1, -1.0))) return st;
+      first = 0;
+    }
+    if (first) return ACCFFT_AMD_ERR_NULL;
+    return ACCFFT_AMD_OK;
+  }
+  static int adjoint_div(accfft_amd_plan* p, void* ox, void* oy, void* oz, const void* v) {
+    void* o[3] = {ox, oy, oz};
+    int st;
+    for (int d = 0; d < 3; ++d)
+      if (o[d])
+        if ((st = deriv(p, d, v, o[d], 0, -1.0))) return st;
+    return ACCFFT_AMD_OK;
   }
 
   // ---------------------------------------------------------- operators

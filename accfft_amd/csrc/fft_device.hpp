@@ -209,25 +209,40 @@ template <> struct Sched<1024> { static constexpr int S = 3; static constexpr in
 template <> struct Sched<2048> { static constexpr int S = 3; static constexpr int R[3] = {16, 16, 8}; };
 template <> struct Sched<4096> { static constexpr int S = 3; static constexpr int R[3] = {16, 16, 16}; };
 
-template <int N>
+// Schedules used with the split-precision matrix-core stages: as many
+// radix-16 stages as possible, the remainder first.
+template <int N> struct SchedSplit { static constexpr int S = 0; static constexpr int R[3] = {0, 0, 0}; };
+template <> struct SchedSplit<256> { static constexpr int S = 2; static constexpr int R[3] = {16, 16, 0}; };
+template <> struct SchedSplit<512> { static constexpr int S = 3; static constexpr int R[3] = {2, 16, 16}; };
+template <> struct SchedSplit<1024> { static constexpr int S = 3; static constexpr int R[3] = {4, 16, 16}; };
+template <> struct SchedSplit<2048> { static constexpr int S = 3; static constexpr int R[3] = {8, 16, 16}; };
+template <> struct SchedSplit<4096> { static constexpr int S = 3; static constexpr int R[3] = {16, 16, 16}; };
+
+template <int N, int SPLIT>
+struct SchedOf {
+  static constexpr int S = SPLIT ? SchedSplit<N>::S : Sched<N>::S;
+  static constexpr int r(int i) { return SPLIT ? SchedSplit<N>::R[i] : Sched<N>::R[i]; }
+};
+
+template <int N, int SPLIT = 0>
 __host__ __device__ constexpr int sched_ns(int s) {  // span before stage s
   int ns = 1;
-  for (int i = 0; i < s; ++i) ns *= Sched<N>::R[i];
+  for (int i = 0; i < s; ++i) ns *= SchedOf<N, SPLIT>::r(i);
   return ns;
 }
-template <int N>
+template <int N, int SPLIT = 0>
 __host__ __device__ constexpr int sched_twoff(int s) {  // twiddle table offset of stage s
   int off = 0;
-  for (int i = 1; i < s; ++i) off += (Sched<N>::R[i] - 1) * sched_ns<N>(i);
+  for (int i = 1; i < s; ++i) off += (SchedOf<N, SPLIT>::r(i) - 1) * sched_ns<N, SPLIT>(i);
   return off;
 }
-template <int N>
-__host__ __device__ constexpr int sched_twn() { return sched_twoff<N>(Sched<N>::S); }
-template <int N>
+template <int N, int SPLIT = 0>
+__host__ __device__ constexpr int sched_twn() { return sched_twoff<N, SPLIT>(SchedOf<N, SPLIT>::S); }
+template <int N, int SPLIT = 0>
 __host__ __device__ constexpr int sched_minnb() {  // fewest butterflies in any stage
   int m = N;
-  for (int i = 0; i < Sched<N>::S; ++i)
-    if (N / Sched<N>::R[i] < m) m = N / Sched<N>::R[i];
+  for (int i = 0; i < SchedOf<N, SPLIT>::S; ++i)
+    if (N / SchedOf<N, SPLIT>::r(i) < m) m = N / SchedOf<N, SPLIT>::r(i);
   return m;
 }
 
@@ -273,19 +288,6 @@ __device__ __forceinline__ void sstage(const LineAcc<T, ES>& acc, int q,
     for (int r = 0; r < R; ++r) acc.st(base + r * NS, v[i][r]);
   }
   __syncthreads();
-}
-
-// Full transform of one line held in LDS (natural order in and out).
-// Caller must __syncthreads() before (data and twiddles staged).
-template <typename T, int N, int NT, int SIGN, int ES>
-__device__ __forceinline__ void static_fft(const LineAcc<T, ES>& acc, int q,
-                                           const cpx<T>* __restrict__ tw) {
-  using S = Sched<N>;
-  sstage<T, N, S::R[0], 1, NT, SIGN, 0, ES>(acc, q, tw);
-  if constexpr (S::S > 1)
-    sstage<T, N, S::R[1], sched_ns<N>(1), NT, SIGN, sched_twoff<N>(1), ES>(acc, q, tw);
-  if constexpr (S::S > 2)
-    sstage<T, N, S::R[2], sched_ns<N>(2), NT, SIGN, sched_twoff<N>(2), ES>(acc, q, tw);
 }
 
 // ------------------------------------------------------------------

@@ -20,7 +20,7 @@
 //           (the per-axis r2c / grad_mult / c2r chain of
 //           operators_gpu.txx:172-278 in one pass, optional accumulate)
 #pragma once
-#include "fft_device.hpp"
+#include "fft_mfma.hpp"
 
 namespace afft {
 
@@ -44,23 +44,35 @@ struct FftArgs {
   int accumulate;                    // M_DERIV: out += result
   int xgroup;                        // s: tiles per XCD chunk (see xcd_tile)
   int reverse;                       // process blocks last-to-first
+  int adjw;                          // z R2C: x2, z C2R: x1/2 on bins other than
+                                     // DC / Nyquist (the adjoint weights)
+  // strided C2C only: rows split in blocks of 2^shift, block b at b * B
+  // (row r at (r % 2^shift) * S + (r >> shift) * B); shift 0 = one block.
+  // This is the pack / unpack of a slab exchange fused into the y pass.
+  int ishift, oshift;
+  int64_t Bin, Bout;
 };
 
+__device__ __forceinline__ int64_t row_off(int r, int64_t S, int shift, int64_t B) {
+  if (shift == 0) return (int64_t)r * S;
+  return (int64_t)(r & ((1 << shift) - 1)) * S + (int64_t)(r >> shift) * B;
+}
+
 // -------------------------------------------------------------- configs
-template <int N>
+template <int N, int SPLIT = 0>
 struct ZCfg {
-  static constexpr int NT = sched_minnb<N>();
+  static constexpr int NT = sched_minnb<N, SPLIT>();
   static constexpr int TPB = NT > 256 ? NT : 256;
   static constexpr int LPW = TPB / NT;
 };
 
-template <typename T, int N>
+template <typename T, int N, int SPLIT = 0>
 struct SCfg {
   static constexpr int TPB = 256;
   // keep the tile at <= 64 KiB of LDS and at least one row per thread group
   static constexpr int W0 = (int)(65536 / (N * (int)sizeof(cpx<T>)));
   static constexpr int WMAX = W0 > 64 ? 64 : W0;
-  static constexpr int WMIN = TPB / sched_minnb<N>();
+  static constexpr int WMIN = TPB / sched_minnb<N, SPLIT>();
   static constexpr int W = WMAX > WMIN ? WMAX : WMIN;
   static constexpr int NT = TPB / W;
 };
@@ -139,13 +151,13 @@ __device__ __forceinline__ void stage_twiddles(cpx<T>* dst, const cpx<T>* __rest
 
 // -------------------------------------------------------------- engines
 // N > 0: static engine; N == 0: dynamic engine.
-template <typename T, int N, int NT, int SIGN>
+template <typename T, int N, int NT, int SIGN, int SPLIT = 0>
 __device__ __forceinline__ cpx<T>* zrun(cpx<T>* A, cpx<T>* B, const FftArgs<T>& a,
                                         const cpx<T>* twl, int lpw) {
   if constexpr (N > 0) {
     const int line = threadIdx.x / NT, q = threadIdx.x % NT;
     LineAcc<T, 1> acc{A + line * N};
-    static_fft<T, N, NT, SIGN, 1>(acc, q, twl);
+    static_fft<T, N, NT, SIGN, 1, SPLIT, 256 / NT>(acc, q, twl, A, N);
     return A;
   } else {
     LayZ lay{a.sc.n};
@@ -153,13 +165,13 @@ __device__ __forceinline__ cpx<T>* zrun(cpx<T>* A, cpx<T>* B, const FftArgs<T>& 
   }
 }
 
-template <typename T, int N, int W, int SIGN>
+template <typename T, int N, int W, int SIGN, int SPLIT = 0>
 __device__ __forceinline__ cpx<T>* srun(cpx<T>* A, cpx<T>* B, const FftArgs<T>& a,
                                         const cpx<T>* twl, int w_dyn) {
   if constexpr (N > 0) {
     const int w = threadIdx.x % W, q = threadIdx.x / W;
     LineAcc<T, W> acc{A + w};
-    static_fft<T, N, 256 / W, SIGN, W>(acc, q, twl);
+    static_fft<T, N, 256 / W, SIGN, W, SPLIT, W>(acc, q, twl, A, 1);
     return A;
   } else {
     LayS lay{w_dyn};
@@ -170,25 +182,25 @@ __device__ __forceinline__ cpx<T>* srun(cpx<T>* A, cpx<T>* B, const FftArgs<T>& 
 // ====================================================================
 // Contiguous lines (z)
 // ====================================================================
-template <typename T, int N, int MODE, int SIGN>
-__global__ void __launch_bounds__(N > 0 ? ZCfg<(N > 0 ? N : 16)>::TPB : 256)
-    zfft_kernel(FftArgs<T> a) {
+template <typename T, int N, int MODE, int SIGN, int SPLIT = 0>
+__global__ void __launch_bounds__(256) zfft_kernel(FftArgs<T> a) {
   constexpr bool STATIC = N > 0;
   constexpr int NS_ = STATIC ? N : 16;
-  constexpr int NT = ZCfg<NS_>::NT;
-  constexpr int TPB = STATIC ? ZCfg<NS_>::TPB : 256;
-  constexpr int TWN = STATIC ? sched_twn<NS_>() : 0;
+  using ZC = ZCfg<NS_, SPLIT>;
+  constexpr int NT = ZC::NT;
+  constexpr int TPB = STATIC ? ZC::TPB : 256;
+  constexpr int TWN = STATIC ? sched_twn<NS_, SPLIT>() : 0;
   constexpr bool PAIR = (MODE == M_R2C || MODE == M_C2R || MODE == M_DERIV);
   const int n = STATIC ? N : a.sc.n;
   const int nh = n / 2 + 1;
-  const int lpw = STATIC ? ZCfg<NS_>::LPW : a.lpw;  // complex lines per block
+  const int lpw = STATIC ? ZC::LPW : a.lpw;  // complex lines per block
   const int t = threadIdx.x;
 
   cpx<T>* bufA;
   cpx<T>* bufB;
   const cpx<T>* twl;
   if constexpr (STATIC) {
-    constexpr int LPW = ZCfg<NS_>::LPW;
+    constexpr int LPW = ZC::LPW;
     __shared__ cpx<T> s_buf[LPW * NS_];
     __shared__ cpx<T> s_tw[TWN > 0 ? TWN : 1];
     bufA = s_buf;
@@ -211,7 +223,7 @@ __global__ void __launch_bounds__(N > 0 ? ZCfg<(N > 0 ? N : 16)>::TPB : 256)
   if constexpr (STATIC) {
     // every thread stages its elements in registers so that all of its
     // global loads are in flight together, then fills LDS
-    constexpr int LPW = ZCfg<NS_>::LPW;
+    constexpr int LPW = ZC::LPW;
     constexpr int EP = LPW * NS_ / TPB;  // elements per thread
     constexpr int NH = NS_ / 2 + 1;
     if constexpr (MODE == M_C2R) {
@@ -238,7 +250,13 @@ __global__ void __launch_bounds__(N > 0 ? ZCfg<(N > 0 ? N : 16)>::TPB : 256)
         if (q < NHT) {
           const int p = q / NH, k = q % NH;
           cpx<T> x = A[j], y = B[j];
-          if (k == 0 || 2 * k == NS_) { x.y = (T)0; y.y = (T)0; }
+          if (k == 0 || 2 * k == NS_) {
+            x.y = (T)0;
+            y.y = (T)0;
+          } else if (a.adjw) {
+            x = cscale(x, (T)0.5);
+            y = cscale(y, (T)0.5);
+          }
           bufA[p * NS_ + k] = cpx<T>{x.x - y.y, x.y + y.x};
           if (k != 0 && 2 * k != NS_) bufA[p * NS_ + NS_ - k] = cpx<T>{x.x + y.y, y.x - x.y};
         }
@@ -287,7 +305,13 @@ __global__ void __launch_bounds__(N > 0 ? ZCfg<(N > 0 ? N : 16)>::TPB : 256)
       cpx<T> A = {(T)0, (T)0}, B = {(T)0, (T)0};
       if (ga < nl) A = ldc<T>(a.in, ga * a.idist + k);
       if (gb < nl) B = ldc<T>(a.in, gb * a.idist + k);
-      if (k == 0 || 2 * k == n) { A.y = (T)0; B.y = (T)0; }
+      if (k == 0 || 2 * k == n) {
+        A.y = (T)0;
+        B.y = (T)0;
+      } else if (a.adjw) {
+        A = cscale(A, (T)0.5);
+        B = cscale(B, (T)0.5);
+      }
       bufA[e] = i < nh ? cpx<T>{A.x - B.y, A.y + B.x} : cpx<T>{A.x + B.y, B.x - A.y};
     }
   }
@@ -296,14 +320,14 @@ __global__ void __launch_bounds__(N > 0 ? ZCfg<(N > 0 ? N : 16)>::TPB : 256)
   // ---------------- transform
   cpx<T>* res;
   if constexpr (MODE == M_C2C) {
-    res = zrun<T, N, NT, SIGN>(bufA, bufB, a, twl, lpw);
+    res = zrun<T, N, NT, SIGN, SPLIT>(bufA, bufB, a, twl, lpw);
   } else if constexpr (MODE == M_R2C) {
-    res = zrun<T, N, NT, -1>(bufA, bufB, a, twl, lpw);
+    res = zrun<T, N, NT, -1, SPLIT>(bufA, bufB, a, twl, lpw);
   } else if constexpr (MODE == M_C2R) {
-    res = zrun<T, N, NT, +1>(bufA, bufB, a, twl, lpw);
+    res = zrun<T, N, NT, +1, SPLIT>(bufA, bufB, a, twl, lpw);
   } else {  // M_DERIV and M_MULT (z has no MULT use; treat as DERIV-style symbol)
-    res = zrun<T, N, NT, -1>(bufA, bufB, a, twl, lpw);
-    const int epm = STATIC ? ZCfg<NS_>::LPW * NS_ / TPB : (lpw * n + TPB - 1) / TPB;
+    res = zrun<T, N, NT, -1, SPLIT>(bufA, bufB, a, twl, lpw);
+    const int epm = STATIC ? ZC::LPW * NS_ / TPB : (lpw * n + TPB - 1) / TPB;
 #pragma unroll
     for (int j = 0; j < epm; ++j) {
       const int e = t + j * TPB;
@@ -315,13 +339,13 @@ __global__ void __launch_bounds__(N > 0 ? ZCfg<(N > 0 ? N : 16)>::TPB : 256)
     }
     __syncthreads();
     cpx<T>* other = (res == bufA) ? bufB : bufA;
-    res = zrun<T, N, NT, +1>(res, other, a, twl, lpw);
+    res = zrun<T, N, NT, +1, SPLIT>(res, other, a, twl, lpw);
   }
 
   // ---------------- store
   // static: trip counts are compile-time so the loops unroll
-  constexpr int EPS = STATIC ? ZCfg<NS_>::LPW * NS_ / TPB : 1;
-  constexpr int EPH = STATIC ? (2 * ZCfg<NS_>::LPW * (NS_ / 2 + 1) + TPB - 1) / TPB : 1;
+  constexpr int EPS = STATIC ? ZC::LPW * NS_ / TPB : 1;
+  constexpr int EPH = STATIC ? (2 * ZC::LPW * (NS_ / 2 + 1) + TPB - 1) / TPB : 1;
   const int ep = STATIC ? EPS : (lpw * n + TPB - 1) / TPB;
   const int eph = STATIC ? EPH : (2 * lpw * nh + TPB - 1) / TPB;
   if constexpr (MODE == M_C2C || MODE == M_MULT) {
@@ -348,6 +372,7 @@ __global__ void __launch_bounds__(N > 0 ? ZCfg<(N > 0 ? N : 16)>::TPB : 256)
       cpx<T> o;
       if ((rl & 1) == 0) o = {h * (z1.x + z2.x), h * (z1.y - z2.y)};
       else o = {h * (z1.y + z2.y), h * (z2.x - z1.x)};
+      if (a.adjw && k != 0 && 2 * k != n) o = cscale(o, (T)2);
       stc<T>(a.out, g * a.odist + k, o);
     }
   } else {  // real outputs
@@ -374,13 +399,13 @@ __global__ void __launch_bounds__(N > 0 ? ZCfg<(N > 0 ? N : 16)>::TPB : 256)
 // ====================================================================
 // Strided lines (y, x)
 // ====================================================================
-template <typename T, int N, int MODE, int SIGN>
+template <typename T, int N, int MODE, int SIGN, int SPLIT = 0>
 __global__ void __launch_bounds__(256) sfft_kernel(FftArgs<T> a) {
   constexpr bool STATIC = N > 0;
   constexpr int NS_ = STATIC ? N : 16;
   constexpr int TPB = 256;
-  constexpr int WS = SCfg<T, NS_>::W;
-  constexpr int TWN = STATIC ? sched_twn<NS_>() : 0;
+  constexpr int WS = SCfg<T, NS_, SPLIT>::W;
+  constexpr int TWN = STATIC ? sched_twn<NS_, SPLIT>() : 0;
   const int n = STATIC ? N : a.sc.n;
   const int nh = n / 2 + 1;
   const int W = STATIC ? WS : a.lpw;  // complex columns per tile
@@ -433,7 +458,7 @@ __global__ void __launch_bounds__(256) sfft_kernel(FftArgs<T> a) {
     for (int j = 0; j < RP; ++j) {
       const int r = r0 + j * RS;
       if constexpr (MODE == M_C2C || MODE == M_MULT) {
-        v[j] = va ? ldc<T>(a.in, ia + (int64_t)r * a.Sin) : cpx<T>{(T)0, (T)0};
+        v[j] = va ? ldc<T>(a.in, ia + row_off(r, a.Sin, a.ishift, a.Bin)) : cpx<T>{(T)0, (T)0};
       } else if constexpr (MODE == M_R2C || MODE == M_DERIV) {
         v[j].x = va ? ldr<T>(a.in, ia + (int64_t)r * a.Sin) : (T)0;
         v[j].y = vb ? ldr<T>(a.in, ib + (int64_t)r * a.Sin) : (T)0;
@@ -451,7 +476,7 @@ __global__ void __launch_bounds__(256) sfft_kernel(FftArgs<T> a) {
   } else if constexpr (MODE == M_C2C || MODE == M_MULT) {
     for (int r = r0; r < n; r += rstep) {
       cpx<T> v = {(T)0, (T)0};
-      if (va) v = ldc<T>(a.in, ia + (int64_t)r * a.Sin);
+      if (va) v = ldc<T>(a.in, ia + row_off(r, a.Sin, a.ishift, a.Bin));
       bufA[r * W + w] = v;
     }
   } else if constexpr (MODE == M_R2C || MODE == M_DERIV) {
@@ -479,13 +504,13 @@ __global__ void __launch_bounds__(256) sfft_kernel(FftArgs<T> a) {
   // ---------------- transform
   cpx<T>* res;
   if constexpr (MODE == M_C2C) {
-    res = srun<T, N, WS, SIGN>(bufA, bufB, a, twl, W);
+    res = srun<T, N, WS, SIGN, SPLIT>(bufA, bufB, a, twl, W);
   } else if constexpr (MODE == M_R2C) {
-    res = srun<T, N, WS, -1>(bufA, bufB, a, twl, W);
+    res = srun<T, N, WS, -1, SPLIT>(bufA, bufB, a, twl, W);
   } else if constexpr (MODE == M_C2R) {
-    res = srun<T, N, WS, +1>(bufA, bufB, a, twl, W);
+    res = srun<T, N, WS, +1, SPLIT>(bufA, bufB, a, twl, W);
   } else {
-    res = srun<T, N, WS, -1>(bufA, bufB, a, twl, W);
+    res = srun<T, N, WS, -1, SPLIT>(bufA, bufB, a, twl, W);
 #pragma unroll
     for (int j = 0; j < rp; ++j) {
       const int r = r0 + j * rstep;
@@ -503,7 +528,7 @@ __global__ void __launch_bounds__(256) sfft_kernel(FftArgs<T> a) {
     }
     __syncthreads();
     cpx<T>* other = (res == bufA) ? bufB : bufA;
-    res = srun<T, N, WS, +1>(res, other, a, twl, W);
+    res = srun<T, N, WS, +1, SPLIT>(res, other, a, twl, W);
   }
 
   // ---------------- store
@@ -512,7 +537,7 @@ __global__ void __launch_bounds__(256) sfft_kernel(FftArgs<T> a) {
 #pragma unroll
       for (int j = 0; j < rp; ++j) {
         const int r = r0 + j * rstep;
-        stc<T>(a.out, oa + (int64_t)r * a.Sout, res[r * W + w]);
+        stc<T>(a.out, oa + row_off(r, a.Sout, a.oshift, a.Bout), res[r * W + w]);
       }
     }
   } else if constexpr (MODE == M_R2C) {

@@ -13,7 +13,16 @@ struct AxisTab {
   DynSched sc{};
   const cpx<T>* tw_static = nullptr;  // per-stage table (static engine)
   const cpx<T>* tw_full = nullptr;    // exp(-2 pi i m/n), m < n
+  const cpx<T>* tw_split = nullptr;   // per-stage table of the split schedule
 };
+
+// Lengths with a matrix-core (split-precision) schedule.
+inline bool split_length(int n) { return n >= 256 && n <= 4096 && (n & (n - 1)) == 0; }
+inline bool split_mode(int mode) { return mode == M_C2C || mode == M_R2C || mode == M_C2R; }
+
+// FP32 pass with radix-16 stages on the matrix cores (fft_split.hip).
+hipError_t launch_fft_split(bool strided, int mode, int sign, FftArgs<float> a,
+                            const AxisTab<float>& tab, int split, hipStream_t st);
 
 inline bool static_length(int n) {
   return n >= 16 && n <= 4096 && (n & (n - 1)) == 0;

@@ -19,7 +19,10 @@
 struct accfft_amd_plan {
   int n[3] = {0, 0, 0};
   int fp = ACCFFT_AMD_F64;
-  int kind = 0;  // 1 = r2c, 2 = c2c
+  int kind = 0;  // 1 = r2c, 2 = c2c, 3 = r2c slab share (accfft_amd_plan_dft_3d_r2c_slab)
+  int nprocs = 1;  // slab share: ranks of the 1-D split (n[0] and n[1] are global)
+  accfft_amd_exchange_fn exchange = nullptr;  // slab share: NULL = identity
+  void* exchange_user = nullptr;
   bool inplace = false;
   unsigned flags = 0;
   void* data = nullptr;
@@ -36,6 +39,7 @@ struct accfft_amd_plan {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_ms = 0.f;
   int device = 0;
+  int split = 0;  // afft::SplitKind of the radix-16 stages (FP32 plans)
   // per-launch event profiling (accfft_amd_plan_set_profiling)
   struct ProfKind {
     std::string name;
@@ -95,10 +99,14 @@ inline cpx<T> host_root(long m, long n) {
   return {(T)cosl(a), (T)(-sinl(a))};
 }
 
-inline int static_sched(int n, int R[3]) {
+inline int static_sched(int n, int R[3], int split = 0) {
 #define AFFT_SCHED(NN)                                              \
   case NN:                                                          \
-    for (int i = 0; i < 3; ++i) R[i] = Sched<NN>::R[i];             \
+    for (int i = 0; i < 3; ++i) R[i] = SchedOf<NN, 0>::r(i);        \
+    if (split && SchedSplit<NN>::S > 0) {                           \
+      for (int i = 0; i < 3; ++i) R[i] = SchedSplit<NN>::R[i];      \
+      return SchedSplit<NN>::S;                                     \
+    }                                                               \
     return Sched<NN>::S;
   switch (n) {
     AFFT_SCHED(16)
@@ -141,9 +149,9 @@ inline DynSched dyn_sched(int n) {
 // Builds the per-stage table of the static engine: for stage s >= 1 with
 // span NS and radix R, entries (r-1)*NS + k = exp(-2 pi i r k / (NS R)).
 template <typename T>
-inline std::vector<cpx<T>> static_table(int n) {
+inline std::vector<cpx<T>> static_table(int n, int split = 0) {
   int R[3];
-  const int S = static_sched(n, R);
+  const int S = static_sched(n, R, split);
   std::vector<cpx<T>> tw;
   int ns = R[0];
   for (int s = 1; s < S; ++s) {

@@ -100,6 +100,175 @@ def run_gpu(n, fp, steps, warmup, profile=True):
     return ms, err, prof, copy_gbps
 
 
+SPLIT_FLAGS = {"bf16x2": 0x1000, "bf16x3": 0x2000, "fp16x2": 0x4000}
+
+
+def run_split(n, steps):
+    """512^3 FP32 forward c2c and r2c: VALU butterflies vs split-precision
+    matrix-core radix-16 stages.  Errors are rel-L2 against the FP64 path of
+    this library on the same (FP32-rounded) input; the tests hold that path
+    to <= 1e-12 of the FP64 oracle."""
+    import torch
+    import accfft_amd as af
+
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(20261204)
+    out = {}
+
+    def timeit(fn):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(steps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    def rel(a, b):
+        return (torch.linalg.vector_norm(a.to(b.dtype) - b) / torch.linalg.vector_norm(b)).item()
+
+    # c2c forward: N(0,1) and the large-dynamic-range field N(0,1) * 10^U(-4,4)
+    z = torch.complex(torch.randn(n, generator=g, device=dev), torch.randn(n, generator=g, device=dev))
+    zo = torch.empty_like(z)
+    zref = torch.empty(n, dtype=torch.complex128, device=dev)
+    p64 = af.Plan("c2c", n, zref, zref, af.F64)
+    z64 = z.to(torch.complex128)
+    p64.execute_c2c(-1, z64, zref)
+    mag = torch.pow(10.0, torch.rand(n, generator=g, device=dev) * 8 - 4)
+    zd = z * mag
+    zdref = torch.empty_like(zref)
+    p64.execute_c2c(-1, zd.to(torch.complex128), zdref)
+    del z64
+    # r2c forward
+    x = torch.randn(n, generator=g, device=dev)
+    h = torch.empty((n[0], n[1], n[2] // 2 + 1), dtype=torch.complex64, device=dev)
+    href = torch.empty(h.shape, dtype=torch.complex128, device=dev)
+    q64 = af.Plan("r2c", n, x, href, af.F64)
+    q64.execute_r2c(x.to(torch.float64), href)
+    torch.cuda.synchronize()
+    p64.destroy()
+    q64.destroy()
+
+    for name, flag in [("fp32_valu", 0)] + list(SPLIT_FLAGS.items()):
+        pc = af.Plan("c2c", n, z, zo, af.F32, af.ACCFFT_MEASURE | flag)
+        pr = af.Plan("r2c", n, x, h, af.F32, af.ACCFFT_MEASURE | flag)
+        tc = timeit(lambda: pc.execute_c2c(-1, z, zo))
+        e_c = rel(zo, zref)
+        pc.execute_c2c(-1, zd, zo)
+        torch.cuda.synchronize()
+        e_cd = rel(zo, zdref)
+        tr = timeit(lambda: pr.execute_r2c(x, h))
+        e_r = rel(h, href)
+        out[name] = {"c2c_ms": round(tc, 4), "r2c_ms": round(tr, 4), "rel_l2_c2c": e_c,
+                     "rel_l2_c2c_dynrange": e_cd, "rel_l2_r2c": e_r}
+        pc.destroy()
+        pr.destroy()
+        log(f"  {name:10s} c2c {tc:.4f} ms ({e_c:.2e}, dyn {e_cd:.2e})  r2c {tr:.4f} ms ({e_r:.2e})")
+    base = out["fp32_valu"]
+    for name in SPLIT_FLAGS:
+        o = out[name]
+        o["speedup_c2c_vs_fp32"] = round(base["c2c_ms"] / o["c2c_ms"], 4)
+        o["speedup_r2c_vs_fp32"] = round(base["r2c_ms"] / o["r2c_ms"], 4)
+        o["meets_rel_err_1e-5"] = max(o["rel_l2_c2c"], o["rel_l2_c2c_dynrange"], o["rel_l2_r2c"]) <= 1e-5
+    return out
+
+
+def _timeit(fn, steps, warmup=2):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def run_wide(steps):
+    """The larger shapes: one GPU's 256 x 2048 x 2048 FP32 share of a 2048^3
+    r2c on 8 ranks (exchange = identity), fused (pack in the y store) and in
+    the reference's structure (z+y, separate pack kernel, x), and a 1024^3
+    FP32 round trip, plain and split-BF16."""
+    import torch
+    import accfft_amd as af
+
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(20261205)
+    out = {}
+    # ---- slab share
+    N, P = (2048, 2048, 2048), 8
+    n0, n1, Nh = N[0] // P, N[1] // P, N[2] // 2 + 1
+    R = n0 * N[1] * N[2] * 4
+    C = n0 * N[1] * Nh * 8
+    u = torch.randn((n0, N[1], N[2]), generator=g, device=dev)
+    h = torch.empty((N[0], n1, Nh), dtype=torch.complex64, device=dev)
+    back = torch.empty_like(u)
+    ps = af.Plan("r2c_slab", N, u, h, af.F32, af.ACCFFT_MEASURE, nprocs=P)
+    t_fwd = _timeit(lambda: ps.execute_r2c(u, h), steps)
+
+    def rt():
+        ps.execute_r2c(u, h)
+        ps.execute_c2r(h, back)
+    t_rt = _timeit(rt, steps)
+    err = (torch.linalg.vector_norm(back / float(N[0] * N[1] * N[2]) - u) /
+           torch.linalg.vector_norm(u)).item()
+    ps.destroy()
+    # the reference's structure: z+y r2c on the slab, pack kernel, x c2c
+    zy = torch.empty((n0, N[1], Nh), dtype=torch.complex64, device=dev)
+    pl = af.Plan("r2c", (n0, N[1], N[2]), u, zy, af.F32)
+    px = af.Plan("c2c", (N[0], n1, Nh), h, h, af.F32)
+
+    def ref_fwd():
+        pl.execute_r2c(u, zy, xyz=6)
+        af.local_transpose_col(n0, P, n1 * Nh * 2, n1 * Nh * 2, zy.view(torch.float32), h.view(torch.float32))
+        px.execute_c2c(-1, h, h, xyz=1)
+    t_ref = _timeit(ref_fwd, steps)
+    pl.destroy()
+    px.destroy()
+    del zy, back
+    out["slab_256x2048x2048_p8"] = {
+        "fwd_ms": round(t_fwd, 4), "roundtrip_ms": round(t_rt, 4), "roundtrip_rel_l2": err,
+        "fwd_pass_model_bytes_fused": R + 5 * C,
+        "fwd_frac_fused_model": round((R + 5 * C) / (t_fwd * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "fwd_frac_reference_model": round((R + 9 * C) / (t_fwd * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "reference_structure_fwd_ms": round(t_ref, 4),
+    }
+    log(f"  slab 256x2048x2048 P=8: fwd {t_fwd:.3f} ms (ref structure {t_ref:.3f} ms), "
+        f"round trip {t_rt:.3f} ms, rel err {err:.2e}")
+    del u, h
+    torch.cuda.empty_cache()
+    # ---- 1024^3 round trip, plain and split-BF16
+    n = (1024, 1024, 1024)
+    x = torch.randn(n, generator=g, device=dev)
+    spec = torch.empty((n[0], n[1], n[2] // 2 + 1), dtype=torch.complex64, device=dev)
+    back = torch.empty_like(x)
+    alg, _ = alg_bytes_roundtrip(n, 4)
+    for name, flag in (("fp32_valu", 0), ("bf16x2", SPLIT_FLAGS["bf16x2"])):
+        p = af.Plan("r2c", n, x, spec, af.F32, af.ACCFFT_MEASURE | flag)
+
+        def step():
+            p.execute_r2c(x, spec)
+            p.execute_c2r(spec, back)
+        t = _timeit(step, max(3, steps // 2))
+        e = (torch.linalg.vector_norm(back / float(n[0] * n[1] * n[2]) - x) /
+             torch.linalg.vector_norm(x)).item()
+        p.destroy()
+        out[f"r2c_c2r_1024_{name}"] = {"ms": round(t, 4), "roundtrip_rel_l2": e,
+                                       "pass_model_frac": round(alg / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"  1024^3 {name}: round trip {t:.3f} ms, rel err {e:.2e}")
+    return out
+
+
 def cpu_baseline(n, budget_s=30.0):
     """FP64 oracle round trip on the host, all allowed threads."""
     import numpy as np
@@ -142,6 +311,9 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 side measurement")
+    ap.add_argument("--no-split", action="store_true", help="skip the split-precision leg")
+    ap.add_argument("--no-wide", action="store_true",
+                    help="skip the slab-share and 1024^3 legs")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -172,6 +344,15 @@ def main():
         extra = {"fp64_ms_per_step": round(ms64, 4), "fp64_rel_err": err64,
                  "fp64_pass_model_frac": round(alg64 / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         log(f"fp64 {n} round trip {ms64:.4f} ms, rel err {err64:.3e}")
+
+    split = None
+    if not args.no_split:
+        log("split-precision matrix-core stages vs FP32 VALU (forward transforms):")
+        split = run_split(n, max(5, args.steps // 2))
+        extra["split_precision"] = split
+    if not args.no_wide:
+        log("larger shapes:")
+        extra["wide"] = run_wide(max(5, args.steps // 2))
 
     out = {
         "metric": "512^3 r2c+c2r ms & %HBM peak; split-BF16 vs FP32 speedup at rel-err<=1e-5",
@@ -212,6 +393,10 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if split:
+        b = split["bf16x2"]
+        out["split_bf16_speedup_c2c"] = b["speedup_c2c_vs_fp32"]
+        out["split_bf16_rel_err_max"] = max(b["rel_l2_c2c"], b["rel_l2_c2c_dynrange"], b["rel_l2_r2c"])
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(n)

@@ -49,6 +49,14 @@ enum { ACCFFT_AMD_F64 = 0, ACCFFT_AMD_F32 = 1 };
  * include/accfft_common.h:40-42 are accepted and do not change results. */
 #define ACCFFT_AMD_BLOCKING 0x100u       /* synchronise the stream after each call */
 #define ACCFFT_AMD_PRESERVE_INPUT 0x200u /* c2r / inverse c2c keep their input */
+/* FP32 plans only (no counterpart in the reference): radix-16 stages of
+ * power-of-two lengths 256..4096 run on the matrix cores with operands split
+ * into 16-bit terms and FP32 accumulation.  BF16: 2 bf16 terms (~16 bits);
+ * BF16X3: 3 bf16 terms (~24 bits); FP16: 2 fp16 terms (~22 bits), each
+ * butterfly scaled by a power of two.  Other lengths keep the FP32 path. */
+#define ACCFFT_AMD_SPLIT_BF16 0x1000u
+#define ACCFFT_AMD_SPLIT_BF16X3 0x2000u
+#define ACCFFT_AMD_SPLIT_FP16 0x4000u
 
 /* Directions (include/accfft_common.h:37-39). */
 #define ACCFFT_AMD_FORWARD (-1)
@@ -80,6 +88,32 @@ int accfft_amd_plan_dft_3d_r2c(const int n[3], void* data, void* data_out, int f
 /* accfft_plan_dft_3d_c2c_gpu(f) (src/accfft_gpu.cpp:704-931). */
 int accfft_amd_plan_dft_3d_c2c(const int n[3], void* data, void* data_out, int fp,
                                unsigned flags, accfft_amd_plan** plan);
+/* One rank's share of accfft_plan_dft_3d_r2c_gpu(f) on a nprocs x 1 grid
+ * (src/accfft_gpu.cpp:77-442 with c_dims = {nprocs, 1}): n is the global
+ * size; the input is the x-slab [n0/nprocs][n1][n2] real, the output the
+ * y-slab [n0][n1/nprocs][n2/2+1] complex, sizes as
+ * accfft_local_size_dft_r2c_gpu (src/accfft_common.cpp:98-192) at coords 0.
+ * execute_r2c runs z r2c, y c2c whose store is the pack of T_plan_2
+ * (local_transpose_col, src/transpose_gpu.cpp:3889-3891), the exchange
+ * (src/transpose_gpu.cpp:3940-3955) through the plan's exchange hook, and
+ * the x c2c on the received blocks; execute_c2r runs the inverse (x c2c in
+ * place on its input, exchange, y c2c whose load is the unpack
+ * local_transpose of src/transpose_gpu.cpp:1411-1413, z c2r).  With no hook
+ * the exchange is the identity (the packed blocks are taken as received):
+ * the per-GPU kernels of the distributed transform at their real shapes.
+ * Requires n0 % nprocs == 0, n1 % nprocs == 0, n1/nprocs a power of two,
+ * and data != data_out (out of place); full xyz only. */
+int accfft_amd_plan_dft_3d_r2c_slab(const int n[3], int nprocs, void* data, void* data_out,
+                                    int fp, unsigned flags, accfft_amd_plan** plan);
+/* Exchange hook of a slab plan (the all-to-all of T_plan_2, no counterpart
+ * as a separate call in the reference): moves nprocs blocks of block_bytes
+ * from send to recv, block q to rank q, on the given stream (a hipStream_t);
+ * direction -1 in execute_r2c, +1 in execute_c2r.  Returns a status.  A
+ * RCCL all-to-all over device buffers is the intended implementation. */
+typedef int (*accfft_amd_exchange_fn)(const void* send, void* recv, int64_t block_bytes,
+                                      int nprocs, int direction, void* stream, void* user);
+int accfft_amd_plan_set_exchange(accfft_amd_plan* plan, accfft_amd_exchange_fn fn,
+                                 void* user);
 /* accfft_destroy_plan_gpu(f) (src/accfft_gpu.cpp:1150-1192).  Also frees the
  * plan object itself (the reference leaks it). */
 int accfft_amd_destroy_plan(accfft_amd_plan* plan);
@@ -87,7 +121,7 @@ int accfft_amd_destroy_plan(accfft_amd_plan* plan);
 int accfft_amd_plan_set_stream(accfft_amd_plan* plan, void* stream);
 /* Plan facts the drop-in wrappers mirror into accfft_plan_gpu(f):
  * n, isize, istart, osize, ostart (3 each), alloc_max, fp, kind (1 = r2c,
- * 2 = c2c), inplace. */
+ * 2 = c2c, 3 = r2c slab share), inplace. */
 int accfft_amd_plan_info(const accfft_amd_plan* plan, int n[3], int isize[3],
                          int istart[3], int osize[3], int ostart[3],
                          int64_t* alloc_max, int* fp, int* kind, int* inplace);
@@ -153,6 +187,27 @@ int accfft_amd_mult_wave_number(int kind, int fp, void* wA, const void* A,
 int accfft_amd_axpy(int fp, int64_t n, double alpha, const void* x, void* y,
                     void* stream);
 
+/* ---------------------------------------------------------------- adjoints */
+/* Adjoints of the r2c plan's real-linear maps (no counterpart in the
+ * reference), with <a, b> = Re sum conj(a) b on half spectra and the dot
+ * product on real fields.  None overwrites its input.
+ *   r2c^T(G) = c2r(w G), w = 1/2 on z bins other than DC / Nyquist;
+ *   c2r^T(y) = r2c(y) / w;
+ *   grad^T(gx, gy, gz) = -div (NULL components are skipped);
+ *   div^T(v) = -grad(v) (NULL outputs are skipped);
+ * laplace, biharmonic and the inverse laplace / biharmonic are self-adjoint
+ * (their adjoint is the forward call). */
+int accfft_amd_adjoint_r2c(accfft_amd_plan* plan, const void* G, void* x, double* timer);
+int accfft_amd_adjoint_c2r(accfft_amd_plan* plan, const void* y, void* G, double* timer);
+int accfft_amd_adjoint_grad(accfft_amd_plan* plan, void* out, const void* gx,
+                            const void* gy, const void* gz, double* timer);
+int accfft_amd_adjoint_divergence(accfft_amd_plan* plan, void* ox, void* oy, void* oz,
+                                  const void* v, double* timer);
+/* Frees the plan's workspaces (intermediate spectra of the operators); they
+ * are reallocated on next use.  Lets a caller trade recomputation for memory. */
+int accfft_amd_plan_release_workspace(accfft_amd_plan* plan);
+int accfft_amd_plan_workspace_bytes(const accfft_amd_plan* plan, int64_t* bytes);
+
 /* ---------------------------------------------------------------- transposes */
 /* Local (on-GPU) transposes of src/transpose_cuda.cu, bit-exact
  * permutations of tuples of elem_size-byte elements.
@@ -182,6 +237,13 @@ int accfft_amd_plan_sync(accfft_amd_plan* plan);
 /* Time of the last kernel launched by this plan's most recent call with a
  * timer, for benchmarking: device milliseconds of [first, last] launch. */
 int accfft_amd_last_elapsed_ms(const accfft_amd_plan* plan, float* ms);
+
+/* Self-test of the matrix-core radix-16 DFT (no counterpart in the
+ * reference): one wavefront transforms 32 vectors of 16 complex FP32 values
+ * (x, y: 512 complex each, vector-major) with split kind 1 (BF16), 2
+ * (BF16X3) or 3 (FP16) and sign -1 / +1.  Pins the matrix-core lane layout
+ * and the split error in the tests. */
+int accfft_amd_selftest_dft16(int split, int sign, const void* x, void* y, void* stream);
 
 /* Per-launch profiling (no counterpart in the reference, whose only
  * instrumentation is the timer[] array, doc/timing.md).  When enabled, every

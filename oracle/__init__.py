@@ -230,6 +230,69 @@ def divergence(ax, ay, az, slow: bool = False):
     return out
 
 
+# ---------------------------------------------------------------- slab share
+def slab_r2c(a, P: int) -> np.ndarray:
+    """One rank's share of the r2c on a P x 1 grid with the exchange taken as
+    the identity (src/accfft_gpu.cpp:471-539; T_plan_2 pack of
+    src/transpose_gpu.cpp:3889-3891): z and y transforms of the x-slab
+    ``a`` [n0][N1][N2], the pack local_transpose_col(n0, P, n1*Nh), then the
+    x transform of the received [P*n0][n1][Nh]."""
+    a = _r(a)
+    n0, N1, N2 = a.shape
+    n1, Nh = N1 // P, N2 // 2 + 1
+    h = r2c(a, xyz=6)
+    packed = local_transpose_col(n0, P, n1 * Nh, n1 * Nh, h.ravel()).reshape(P * n0, n1, Nh)
+    return c2c(packed, -1, xyz=1)
+
+
+def slab_c2r(h, N2: int, P: int) -> np.ndarray:
+    """The inverse: x transform of [N0][n1][Nh], the unpack
+    local_transpose(P, n0, n1*Nh) (src/transpose_gpu.cpp:1411-1413), y and z
+    c2r of the x-slab [n0][N1][N2]."""
+    h = _c(h)
+    N0, n1, Nh = h.shape
+    n0 = N0 // P
+    x = c2c(h, +1, xyz=1)
+    un = local_transpose(P, n0, n1 * Nh, x.ravel()).reshape(n0, P * n1, Nh)
+    return c2r(un, (n0, P * n1, N2), xyz=6)
+
+
+# ---------------------------------------------------------------- adjoints
+# No counterpart in the reference: adjoints of its real-linear maps for
+# reverse-mode differentiation, with <a, b> = Re sum conj(a) b on half
+# spectra.  Pinned in tests/test_oracle.py against explicit transposed
+# matrices built column by column from r2c / c2r / grad / divergence.
+def adjoint_weight(n) -> np.ndarray:
+    """w[k] over the z half spectrum: 1 at DC and (even n[2]) Nyquist, 1/2 elsewhere."""
+    n2 = int(n[2])
+    w = np.full(n2 // 2 + 1, 0.5)
+    w[0] = 1.0
+    if n2 % 2 == 0:
+        w[-1] = 1.0
+    return w
+
+
+def adjoint_r2c(G, n) -> np.ndarray:
+    """r2c^T(G) = c2r(w G)."""
+    return c2r(_c(G) * adjoint_weight(n), n)
+
+
+def adjoint_c2r(y) -> np.ndarray:
+    """c2r^T(y) = r2c(y) / w."""
+    y = _r(y)
+    return r2c(y) / adjoint_weight(y.shape)
+
+
+def adjoint_grad(gx, gy, gz) -> np.ndarray:
+    """grad^T(gx, gy, gz) = -div(gx, gy, gz) (per-axis derivatives)."""
+    return -divergence(gx, gy, gz)
+
+
+def adjoint_divergence(v):
+    """div^T(v) = -grad(v) (per-axis derivatives)."""
+    return tuple(-g for g in grad(v))
+
+
 # ---------------------------------------------------------------- transposes
 def local_transpose(r, c, t, a) -> np.ndarray:
     """[r][c][t] -> [c][r][t]; src/transpose.cpp:664-679 (cuda1/cuda3)."""

@@ -226,3 +226,96 @@ def test_div_grad_invlap_projection(orc):
             for k in (0, 5):
                 mask[i, j, k] = 0
     assert rel_l2(lhs, orc.c2r(h * mask, n) / np.prod(n)) < 1e-13
+
+
+# ---------------------------------------------------------------- adjoints
+# Each adjoint formula is checked against the explicit transpose of the map's
+# matrix, assembled column by column from the forward oracle on tiny grids
+# (odd and even lengths on every axis, so DC-only and DC+Nyquist z planes).
+ADJ_SHAPES = [(4, 3, 6), (3, 4, 5), (2, 5, 4)]
+
+
+def _basis(n):
+    N = int(np.prod(n))
+    for j in range(N):
+        e = np.zeros(N)
+        e[j] = 1.0
+        yield j, e.reshape(n)
+
+
+@pytest.mark.parametrize("n", ADJ_SHAPES)
+def test_adjoint_r2c_is_matrix_transpose(orc, n):
+    N = int(np.prod(n))
+    M = np.stack([orc.r2c(e).ravel() for _, e in _basis(n)], axis=1)  # complex [K][N]
+    g = rng(40)
+    G = (g.standard_normal(M.shape[0]) + 1j * g.standard_normal(M.shape[0]))
+    expect = (M.real.T @ G.real + M.imag.T @ G.imag).reshape(n)
+    assert rel_l2(orc.adjoint_r2c(G.reshape(n[0], n[1], n[2] // 2 + 1), n), expect) < 1e-13
+    assert M.shape == (n[0] * n[1] * (n[2] // 2 + 1), N)
+
+
+@pytest.mark.parametrize("n", ADJ_SHAPES)
+def test_adjoint_c2r_is_matrix_transpose(orc, n):
+    hs = (n[0], n[1], n[2] // 2 + 1)
+    K = int(np.prod(hs))
+    cols_re, cols_im = [], []
+    for k in range(K):
+        e = np.zeros(K, np.complex128)
+        e[k] = 1.0
+        cols_re.append(orc.c2r(e.reshape(hs), n).ravel())
+        e[k] = 1j
+        cols_im.append(orc.c2r(e.reshape(hs), n).ravel())
+    y = rng(41).standard_normal(int(np.prod(n)))
+    expect = (np.array(cols_re) @ y + 1j * (np.array(cols_im) @ y)).reshape(hs)
+    assert rel_l2(orc.adjoint_c2r(y.reshape(n)), expect) < 1e-13
+
+
+@pytest.mark.parametrize("n", ADJ_SHAPES)
+def test_adjoint_grad_div_are_matrix_transposes(orc, n):
+    cols = [[], [], []]
+    for _, e in _basis(n):
+        for d, gd in enumerate(orc.grad(e)):
+            cols[d].append(gd.ravel())
+    Md = [np.array(c).T for c in cols]  # [N][N] per component
+    g = rng(42)
+    gs = [g.standard_normal(n) for _ in range(3)]
+    expect = sum(Md[d].T @ gs[d].ravel() for d in range(3)).reshape(n)
+    assert rel_l2(orc.adjoint_grad(*gs), expect) < 1e-12
+    # divergence is the row block [Dx Dy Dz]; its transpose is (Dx^T v, Dy^T v, Dz^T v)
+    v = g.standard_normal(n)
+    for d, o in enumerate(orc.adjoint_divergence(v)):
+        assert rel_l2(o, (Md[d].T @ v.ravel()).reshape(n)) < 1e-12
+    # and the divergence oracle is that row block
+    assert rel_l2(orc.divergence(*gs), sum(Md[d] @ gs[d].ravel() for d in range(3)).reshape(n)) < 1e-12
+
+
+@pytest.mark.parametrize("op", ["laplace", "inv_laplace", "biharmonic", "inv_biharmonic"])
+@pytest.mark.parametrize("n", ADJ_SHAPES)
+def test_spectral_ops_are_symmetric(orc, op, n):
+    M = np.array([getattr(orc, op)(e).ravel() for _, e in _basis(n)]).T
+    assert np.abs(M - M.T).max() <= 1e-12 * max(1.0, np.abs(M).max())
+
+
+# ---------------------------------------------------------------- slab share
+def test_slab_one_rank_is_plain_r2c(orc):
+    a = rng(60).standard_normal((6, 8, 10))
+    assert rel_l2(orc.slab_r2c(a, 1), orc.r2c(a)) < 1e-14
+    h = orc.r2c(a)
+    assert rel_l2(orc.slab_c2r(h, 10, 1), orc.c2r(h, (6, 8, 10))) < 1e-14
+
+
+@pytest.mark.parametrize("shape,P", [((8, 16, 10), 4), ((12, 6, 9), 3), ((16, 8, 8), 8)])
+def test_slab_round_trip_and_blocks(orc, shape, P):
+    N0, N1, N2 = shape
+    n0 = N0 // P
+    a = rng(61).standard_normal((n0, N1, N2))
+    h = orc.slab_r2c(a, P)
+    assert h.shape == (N0, N1 // P, N2 // 2 + 1)
+    assert rel_l2(orc.slab_c2r(h, N2, P), N0 * N1 * N2 * a) < 1e-13
+    # block q of the received array (before the x transform) is the y-range
+    # q of the z/y-transformed slab: check through the x-inverse
+    zy = np.fft.fft(np.fft.rfft(a, axis=2), axis=1)
+    recv = np.fft.ifft(h, axis=0)
+    n1 = N1 // P
+    for q in range(P):
+        assert rel_l2(recv[q * n0:(q + 1) * n0], zy[:, q * n1:(q + 1) * n1]) < 1e-13

@@ -127,6 +127,17 @@ def test_errors_without_gpu(af):
     with pytest.raises(af.AccfftError):
         af.local_size_dft_r2c((8, 8, 8), af.F64, (2, 2), (2, 0))
     assert L.accfft_amd_status_string(3) == b"unsupported transform length"
+    # slab share: the split must divide n0 and n1, n1/P a power of two, out of place
+    slab = L.accfft_amd_plan_dft_3d_r2c_slab
+    a, b = ctypes.c_void_p(16), ctypes.c_void_p(32)
+    assert slab(i3(64, 64, 64), 0, a, b, af.F64, 0, ctypes.byref(h)) == 1
+    assert slab(i3(60, 64, 64), 8, a, b, af.F64, 0, ctypes.byref(h)) == 3
+    assert slab(i3(64, 60, 64), 8, a, b, af.F64, 0, ctypes.byref(h)) == 3
+    assert slab(i3(48, 48, 64), 4, a, b, af.F64, 0, ctypes.byref(h)) == 3  # n1/P = 12
+    assert slab(i3(64, 64, 64), 8, a, a, af.F64, 0, ctypes.byref(h)) == 3  # in place
+    assert slab(i3(64, 64, 64), 8, None, None, af.F64, 0, ctypes.byref(h)) == 3
+    assert not h.value
+    assert L.accfft_amd_plan_set_exchange(None, af.EXCHANGE_FN(), None) == 7
 
 
 def test_transpose_argument_checks(af):
