@@ -66,11 +66,11 @@ struct ZCfg {
   static constexpr int LPW = TPB / NT;
 };
 
-template <typename T, int N, int SPLIT = 0>
+template <typename T, int N, int SPLIT = 0, int LDSK = 64>
 struct SCfg {
   static constexpr int TPB = 256;
-  // keep the tile at <= 64 KiB of LDS and at least one row per thread group
-  static constexpr int W0 = (int)(65536 / (N * (int)sizeof(cpx<T>)));
+  // keep the tile at <= LDSK KiB of LDS and at least one row per thread group
+  static constexpr int W0 = (int)(LDSK * 1024 / (N * (int)sizeof(cpx<T>)));
   static constexpr int WMAX = W0 > 64 ? 64 : W0;
   static constexpr int WMIN = TPB / sched_minnb<N, SPLIT>();
   static constexpr int W = WMAX > WMIN ? WMAX : WMIN;
@@ -399,12 +399,12 @@ __global__ void __launch_bounds__(256) zfft_kernel(FftArgs<T> a) {
 // ====================================================================
 // Strided lines (y, x)
 // ====================================================================
-template <typename T, int N, int MODE, int SIGN, int SPLIT = 0>
+template <typename T, int N, int MODE, int SIGN, int SPLIT = 0, int LDSK = 64>
 __global__ void __launch_bounds__(256) sfft_kernel(FftArgs<T> a) {
   constexpr bool STATIC = N > 0;
   constexpr int NS_ = STATIC ? N : 16;
   constexpr int TPB = 256;
-  constexpr int WS = SCfg<T, NS_, SPLIT>::W;
+  constexpr int WS = SCfg<T, NS_, SPLIT, LDSK>::W;
   constexpr int TWN = STATIC ? sched_twn<NS_, SPLIT>() : 0;
   const int n = STATIC ? N : a.sc.n;
   const int nh = n / 2 + 1;

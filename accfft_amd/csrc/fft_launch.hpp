@@ -99,6 +99,20 @@ inline void dyn_lds_attrs() {
   (void)hipFuncSetAttribute((const void*)sfft_kernel<T, 0, M_MULT, -1>, hipFuncAttributeMaxDynamicSharedMemorySize, cap);
 }
 
+// Strided C2C tiles of half the LDS (32 KiB: twice the workgroups per CU,
+// half-width row segments) for the lengths whose bit log2(n) is set in
+// ACCFFT_AMD_SNARROW (read once; default none).
+inline bool narrow_tiles(int n) {
+  static const unsigned mask = [] {
+    const char* e = getenv("ACCFFT_AMD_SNARROW");
+    return e ? (unsigned)strtoul(e, nullptr, 0) : 0u;
+  }();
+  if (n != 512 && n != 1024 && n != 2048) return false;
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  return (mask >> l) & 1u;
+}
+
 // Launch one axis pass.  strided = false: contiguous lines (z layout).
 // Returns hipErrorNotSupported when the length does not fit one LDS tile.
 template <typename T>
@@ -113,6 +127,23 @@ hipError_t launch_fft(bool strided, int mode, int sign, FftArgs<T> a, const Axis
   if (tab.is_static) {
     a.tw = tab.tw_static;
     int per;
+    if (strided && mode == M_C2C && narrow_tiles(n)) {
+      const int w = n == 512 ? SCfg<T, 512, 0, 32>::W
+                  : n == 1024 ? SCfg<T, 1024, 0, 32>::W : SCfg<T, 2048, 0, 32>::W;
+      a.lpw = w;
+      const int64_t grid = (units + w - 1) / w;
+      dim3 g((unsigned)grid), b(256);
+#define AFFT_NARROW(NN)                                                                  \
+  if (n == NN) {                                                                         \
+    if (sign < 0) hipLaunchKernelGGL((sfft_kernel<T, NN, M_C2C, -1, 0, 32>), g, b, 0, st, a); \
+    else hipLaunchKernelGGL((sfft_kernel<T, NN, M_C2C, +1, 0, 32>), g, b, 0, st, a);      \
+    return hipGetLastError();                                                            \
+  }
+      AFFT_NARROW(512)
+      AFFT_NARROW(1024)
+      AFFT_NARROW(2048)
+#undef AFFT_NARROW
+    }
 #define AFFT_CASE(NN)                                                              \
   case NN:                                                                         \
     per = strided ? SCfg<T, NN>::W : ZCfg<NN>::LPW;                                \
