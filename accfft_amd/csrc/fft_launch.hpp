@@ -113,6 +113,19 @@ inline bool narrow_tiles(int n) {
   return (mask >> l) & 1u;
 }
 
+// Strided C2C tiles of 128 KiB of LDS (one workgroup per CU, twice the row
+// segment) for the lengths whose bit log2(n) is set in ACCFFT_AMD_SWIDE.
+inline bool wide_tiles(int n) {
+  static const unsigned mask = [] {
+    const char* e = getenv("ACCFFT_AMD_SWIDE");
+    return e ? (unsigned)strtoul(e, nullptr, 0) : 0u;
+  }();
+  if (n != 1024 && n != 2048 && n != 4096) return false;
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  return (mask >> l) & 1u;
+}
+
 // Launch one axis pass.  strided = false: contiguous lines (z layout).
 // Returns hipErrorNotSupported when the length does not fit one LDS tile.
 template <typename T>
@@ -127,6 +140,24 @@ hipError_t launch_fft(bool strided, int mode, int sign, FftArgs<T> a, const Axis
   if (tab.is_static) {
     a.tw = tab.tw_static;
     int per;
+    if (sizeof(T) == 4 && strided && mode == M_C2C && wide_tiles(n)) {
+      const int w = n == 1024 ? SCfg<T, 1024, 0, 128>::W
+                  : n == 2048 ? SCfg<T, 2048, 0, 128>::W : SCfg<T, 4096, 0, 128>::W;
+      a.lpw = w;
+      const int64_t grid = (units + w - 1) / w;
+      dim3 g((unsigned)grid), b(256);
+#define AFFT_WIDE(NN)                                                                     \
+  if constexpr (sizeof(T) == 4)                                                           \
+    if (n == NN) {                                                                        \
+      if (sign < 0) hipLaunchKernelGGL((sfft_kernel<T, NN, M_C2C, -1, 0, 128>), g, b, 0, st, a); \
+      else hipLaunchKernelGGL((sfft_kernel<T, NN, M_C2C, +1, 0, 128>), g, b, 0, st, a);    \
+      return hipGetLastError();                                                           \
+    }
+      AFFT_WIDE(1024)
+      AFFT_WIDE(2048)
+      AFFT_WIDE(4096)
+#undef AFFT_WIDE
+    }
     if (strided && mode == M_C2C && narrow_tiles(n)) {
       const int w = n == 512 ? SCfg<T, 512, 0, 32>::W
                   : n == 1024 ? SCfg<T, 1024, 0, 32>::W : SCfg<T, 2048, 0, 32>::W;
