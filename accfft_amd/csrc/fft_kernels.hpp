@@ -571,6 +571,143 @@ __global__ void __launch_bounds__(256) sfft_kernel(FftArgs<T> a) {
 }
 
 // ====================================================================
+// Strided C2C, two-level (four-step) tiles for long lines
+// ====================================================================
+// N = R1 * M.  A tile of W columns is held in registers after the first
+// level: thread (c, n2) loads x[n2 + M n1] for n1 < R1 straight from HBM,
+// does the R1-point DFT and the twiddle exp(-+2 pi i n2 k1 / N) in
+// registers; the M-point DFTs over n2 (one per k1 and column) then run in
+// LDS, KC values of k1 at a time, and store X[k1 + R1 k2].  Only a
+// KC/R1 share of the tile is ever in LDS, so W (the row segment) is 2-4x
+// wider than one-level tiles of the same LDS budget.
+// Variant V = 0 is the default of each length; V = 1 an alternative kept
+// for measurement (ACCFFT_AMD_S4ALT).
+template <typename T, int N, int V = 0>
+struct S4Cfg {
+  static constexpr bool OK = false;
+  static constexpr int R1 = 16, W = 1, TPB = 256, KC = 1;
+};
+#define AFFT_S4CFG(TT, NN, VV, R1_, W_, TPB_, KC_)         \
+  template <>                                              \
+  struct S4Cfg<TT, NN, VV> {                               \
+    static constexpr bool OK = true;                       \
+    static constexpr int R1 = R1_, W = W_, TPB = TPB_, KC = KC_; \
+  };
+AFFT_S4CFG(float, 512, 0, 16, 16, 512, 8)
+AFFT_S4CFG(float, 512, 1, 16, 32, 1024, 8)
+AFFT_S4CFG(float, 1024, 0, 16, 8, 512, 8)
+AFFT_S4CFG(float, 1024, 1, 16, 16, 1024, 8)
+AFFT_S4CFG(float, 2048, 0, 16, 8, 1024, 16)
+AFFT_S4CFG(float, 2048, 1, 16, 4, 512, 16)
+AFFT_S4CFG(double, 512, 0, 16, 16, 512, 8)
+AFFT_S4CFG(double, 512, 1, 16, 8, 256, 16)
+#undef AFFT_S4CFG
+
+template <typename T, int N, int SIGN, bool BLK, int V>
+__global__ void __launch_bounds__((S4Cfg<T, N, V>::TPB))
+    sfft4_kernel(FftArgs<T> a, const cpx<T>* __restrict__ twsub) {
+  using CF = S4Cfg<T, N, V>;
+  constexpr int R1 = CF::R1, W = CF::W, TPB = CF::TPB, KC = CF::KC;
+  constexpr int M = N / R1;
+  constexpr int D1 = W * M / TPB;   // first-level DFTs per thread
+  constexpr int RS1 = TPB / W;      // n2 step between them
+  constexpr int NT = TPB / (KC * W);  // threads per M-point line
+  constexpr int TWM = sched_twn<M, 0>();
+  static_assert(D1 >= 1 && D1 * TPB == W * M, "tile must split evenly");
+  static_assert(NT >= 1 && NT * KC * W == TPB, "lines must split evenly");
+  static_assert(R1 % KC == 0, "k1 chunks must split evenly");
+  __shared__ cpx<T> buf[KC * M * W];
+  __shared__ cpx<T> tw1[N];
+  __shared__ cpx<T> tw2[TWM > 0 ? TWM : 1];
+  const int t = threadIdx.x;
+  const int c = t % W, r0 = t / W;
+  const int64_t G = a.nlines;
+  int64_t tile = xcd_tile(blockIdx.x, gridDim.x, (unsigned)a.xgroup);
+  if (a.reverse) tile = (int64_t)gridDim.x - 1 - tile;
+  const int64_t col = tile * W + c;
+  const bool ok = col < G;
+  const int64_t ib = (col / a.C) * a.Din + (col % a.C);
+  const int64_t ob = (col / a.C) * a.Dout + (col % a.C);
+
+  // twiddle loads first, so that waiting for them does not wait for the data
+  constexpr int E1 = (N + TPB - 1) / TPB, E2 = (TWM + TPB - 1) / TPB;
+  cpx<T> tv1[E1], tv2[E2 > 0 ? E2 : 1];
+#pragma unroll
+  for (int j = 0; j < E1; ++j)
+    if (t + j * TPB < N) tv1[j] = a.tw[t + j * TPB];
+#pragma unroll
+  for (int j = 0; j < E2; ++j)
+    if (t + j * TPB < TWM) tv2[j] = twsub[t + j * TPB];
+  cpx<T> v[D1][R1];
+  const cpx<T>* src = reinterpret_cast<const cpx<T>*>(a.in) + ib;
+#pragma unroll
+  for (int d = 0; d < D1; ++d) {
+    if constexpr (BLK) {
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) {
+        const int row = r0 + d * RS1 + M * n1;
+        v[d][n1] = ok ? src[row_off(row, a.Sin, a.ishift, a.Bin)] : cpx<T>{(T)0, (T)0};
+      }
+    } else {
+      // running pointer, opaque to the compiler: the R1 row offsets are not
+      // precomputed into registers
+      const cpx<T>* p = src + (int64_t)(r0 + d * RS1) * a.Sin;
+      const int64_t step = (int64_t)M * a.Sin;
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) {
+        v[d][n1] = ok ? *p : cpx<T>{(T)0, (T)0};
+        p += step;
+        asm volatile("" : "+v"(p));
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < E1; ++j)
+    if (t + j * TPB < N) tw1[t + j * TPB] = tv1[j];
+#pragma unroll
+  for (int j = 0; j < E2; ++j)
+    if (t + j * TPB < TWM) tw2[t + j * TPB] = tv2[j];
+  __syncthreads();
+
+  // first level in registers
+#pragma unroll
+  for (int d = 0; d < D1; ++d) {
+    dft<T, R1, SIGN>(v[d]);
+    const int n2 = r0 + d * RS1;
+#pragma unroll
+    for (int k1 = 1; k1 < R1; ++k1) {
+      const cpx<T> w = tw1[n2 * k1];
+      v[d][k1] = (SIGN < 0) ? cmul(v[d][k1], w) : cmulc(v[d][k1], w);
+    }
+  }
+
+  // second level in LDS, KC values of k1 at a time
+  const int q = r0 % NT, kl = r0 / NT;
+#pragma unroll
+  for (int kc0 = 0; kc0 < R1; kc0 += KC) {
+#pragma unroll
+    for (int d = 0; d < D1; ++d)
+#pragma unroll
+      for (int kk = 0; kk < KC; ++kk) buf[(kk * M + r0 + d * RS1) * W + c] = v[d][kc0 + kk];
+    __syncthreads();
+    LineAcc<T, W> acc{buf + kl * M * W + c};
+    static_fft<T, M, NT, SIGN, W, 0, 1>(acc, q, tw2, buf, 0);
+    if (ok) {
+      cpx<T>* dst = reinterpret_cast<cpx<T>*>(a.out) + ob;
+#pragma unroll
+      for (int kk = 0; kk < KC; ++kk)
+#pragma unroll
+        for (int k2 = r0; k2 < M; k2 += RS1) {
+          const int row = kc0 + kk + R1 * k2;
+          const int64_t off = BLK ? row_off(row, a.Sout, a.oshift, a.Bout) : (int64_t)row * a.Sout;
+          dst[off] = buf[(kk * M + k2) * W + c];
+        }
+    }
+    __syncthreads();
+  }
+}
+
+// ====================================================================
 // Element-wise kernels
 // ====================================================================
 // out = symbol(kind) * scale * in over a complex block [osize] at global

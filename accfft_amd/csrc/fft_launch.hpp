@@ -14,6 +14,7 @@ struct AxisTab {
   const cpx<T>* tw_static = nullptr;  // per-stage table (static engine)
   const cpx<T>* tw_full = nullptr;    // exp(-2 pi i m/n), m < n
   const cpx<T>* tw_split = nullptr;   // per-stage table of the split schedule
+  const cpx<T>* tw_sub = nullptr;     // per-stage table of length n/16 (two-level tiles)
 };
 
 // Lengths with a matrix-core (split-precision) schedule.
@@ -126,6 +127,61 @@ inline bool wide_tiles(int n) {
   return (mask >> l) & 1u;
 }
 
+// Two-level strided C2C tiles (sfft4_kernel): on by default where they were
+// measured faster on MI355X (FP32 512, 1024, 2048; FP64 512 is faster with
+// the one-level tiles); the masks ACCFFT_AMD_S4 (bit log2(n): use them) and
+// ACCFFT_AMD_S4ALT (bit: use the alternative configuration) override, for
+// measurement.
+inline unsigned env_mask(const char* name, unsigned dflt) {
+  const char* e = getenv(name);
+  return e ? (unsigned)strtoul(e, nullptr, 0) : dflt;
+}
+inline int ilog2i(int n) {
+  int l = 0;
+  while ((1 << l) < n) ++l;
+  return l;
+}
+template <typename T>
+inline bool four_step(int n) {
+  static const unsigned mask =
+      env_mask("ACCFFT_AMD_S4", sizeof(T) == 4 ? (1u << 9) | (1u << 10) | (1u << 11) : 0u);
+  return (mask >> ilog2i(n)) & 1u;
+}
+inline bool four_step_alt(int n) {
+  static const unsigned mask = env_mask("ACCFFT_AMD_S4ALT", 0u);
+  return (mask >> ilog2i(n)) & 1u;
+}
+
+template <typename T, int N, int V>
+hipError_t launch_s4v(int sign, FftArgs<T>& a, const AxisTab<T>& tab, hipStream_t st) {
+  if constexpr (!S4Cfg<T, N, V>::OK) {
+    return hipErrorNotSupported;
+  } else {
+    using CF = S4Cfg<T, N, V>;
+    const int64_t tiles = (a.nlines + CF::W - 1) / CF::W;
+    a.tw = tab.tw_full;
+    a.lpw = CF::W;
+    dim3 g((unsigned)tiles), b(CF::TPB);
+    const bool blk = a.ishift != 0 || a.oshift != 0;
+#define AFFT_S4(S, B) hipLaunchKernelGGL((sfft4_kernel<T, N, S, B, V>), g, b, 0, st, a, tab.tw_sub)
+    if (blk) {
+      if (sign < 0) AFFT_S4(-1, true);
+      else AFFT_S4(+1, true);
+    } else {
+      if (sign < 0) AFFT_S4(-1, false);
+      else AFFT_S4(+1, false);
+    }
+#undef AFFT_S4
+    return hipGetLastError();
+  }
+}
+
+template <typename T, int N>
+hipError_t launch_s4(int sign, FftArgs<T>& a, const AxisTab<T>& tab, hipStream_t st) {
+  return four_step_alt(N) ? launch_s4v<T, N, 1>(sign, a, tab, st)
+                          : launch_s4v<T, N, 0>(sign, a, tab, st);
+}
+
 // Launch one axis pass.  strided = false: contiguous lines (z layout).
 // Returns hipErrorNotSupported when the length does not fit one LDS tile.
 template <typename T>
@@ -140,6 +196,14 @@ hipError_t launch_fft(bool strided, int mode, int sign, FftArgs<T> a, const Axis
   if (tab.is_static) {
     a.tw = tab.tw_static;
     int per;
+    if (strided && mode == M_C2C && tab.tw_sub && four_step<T>(n)) {
+      switch (n) {
+        case 512: return launch_s4<T, 512>(sign, a, tab, st);
+        case 1024: return launch_s4<T, 1024>(sign, a, tab, st);
+        case 2048: return launch_s4<T, 2048>(sign, a, tab, st);
+        default: break;
+      }
+    }
     if (sizeof(T) == 4 && strided && mode == M_C2C && wide_tiles(n)) {
       const int w = n == 1024 ? SCfg<T, 1024, 0, 128>::W
                   : n == 2048 ? SCfg<T, 2048, 0, 128>::W : SCfg<T, 4096, 0, 128>::W;
