@@ -36,6 +36,22 @@ def alg_bytes_roundtrip(n, s):
     return 2 * R + 10 * C, 2 * (R + C)
 
 
+def profiled_traffic(kernel, n):
+    """HBM bytes per launch of ``kernel`` from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by kernels of known size), or
+    None when no summary covers it."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"fft_passes_{n[0]}cubed_fp32_hbm_traffic.md")
+    if n[0] != n[1] or n[1] != n[2] or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch:" and parts[1] == kernel:
+                return int(parts[2])
+    return None
+
+
 def run_gpu(n, fp, steps, warmup, profile=True):
     import torch
     import accfft_amd as af
@@ -387,7 +403,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": profiled_traffic(dname, n),
             "alg_bytes_per_launch": dbytes,
             "avg_launch_us": round(davg_ms * 1e3, 2),
         },
