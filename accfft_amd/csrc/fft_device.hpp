@@ -218,10 +218,25 @@ template <> struct SchedSplit<1024> { static constexpr int S = 3; static constex
 template <> struct SchedSplit<2048> { static constexpr int S = 3; static constexpr int R[3] = {8, 16, 16}; };
 template <> struct SchedSplit<4096> { static constexpr int S = 3; static constexpr int R[3] = {16, 16, 16}; };
 
+// Schedules with more, smaller radices (more butterflies per stage, so more
+// threads per line): the second level of the two-level strided tiles.
+template <int N> struct SchedWide { static constexpr int S = 0; static constexpr int R[3] = {0, 0, 0}; };
+template <> struct SchedWide<64> { static constexpr int S = 3; static constexpr int R[3] = {4, 4, 4}; };
+template <> struct SchedWide<128> { static constexpr int S = 3; static constexpr int R[3] = {8, 4, 4}; };
+template <> struct SchedWide<256> { static constexpr int S = 3; static constexpr int R[3] = {4, 8, 8}; };
+
+// Schedule id of the static engine: 0 = Sched, SCHED_WIDE = SchedWide,
+// any other value = a split-precision kind (SchedSplit).
+constexpr int SCHED_WIDE = 8;
+__host__ __device__ constexpr bool split_sched(int id) { return id != 0 && id != SCHED_WIDE; }
+
 template <int N, int SPLIT>
 struct SchedOf {
-  static constexpr int S = SPLIT ? SchedSplit<N>::S : Sched<N>::S;
-  static constexpr int r(int i) { return SPLIT ? SchedSplit<N>::R[i] : Sched<N>::R[i]; }
+  static constexpr int S = SPLIT == SCHED_WIDE ? SchedWide<N>::S
+                         : SPLIT ? SchedSplit<N>::S : Sched<N>::S;
+  static constexpr int r(int i) {
+    return SPLIT == SCHED_WIDE ? SchedWide<N>::R[i] : SPLIT ? SchedSplit<N>::R[i] : Sched<N>::R[i];
+  }
 };
 
 template <int N, int SPLIT = 0>

@@ -580,130 +580,258 @@ __global__ void __launch_bounds__(256) sfft_kernel(FftArgs<T> a) {
 // LDS, KC values of k1 at a time, and store X[k1 + R1 k2].  Only a
 // KC/R1 share of the tile is ever in LDS, so W (the row segment) is 2-4x
 // wider than one-level tiles of the same LDS budget.
-// Variant V = 0 is the default of each length; V = 1 an alternative kept
-// for measurement (ACCFFT_AMD_S4ALT).
+// TWF = 1 (R1 = 16) reads only w, w^2, w^4, w^8 of the first-level
+// twiddles w^k1 = exp(-+2 pi i n2 k1 / N) from a table of N/2 entries and
+// forms the other powers with at most three complex products (a few ulp):
+// half the LDS table of TWF = 0 and four live twiddles instead of fifteen.  SUB is the schedule id of
+// the second level (0 or SCHED_WIDE: more threads per M-point line).
+// Variant V = 0 is the default of each length; the others are kept for
+// measurement (selected with ACCFFT_AMD_S4V).
 template <typename T, int N, int V = 0>
 struct S4Cfg {
   static constexpr bool OK = false;
-  static constexpr int R1 = 16, W = 1, TPB = 256, KC = 1;
+  static constexpr int R1 = 16, W = 1, TPB = 256, KC = 1, TWF = 0, SUB = 0, WPE = 1, PB = 0;
 };
-#define AFFT_S4CFG(TT, NN, VV, R1_, W_, TPB_, KC_)         \
-  template <>                                              \
-  struct S4Cfg<TT, NN, VV> {                               \
-    static constexpr bool OK = true;                       \
-    static constexpr int R1 = R1_, W = W_, TPB = TPB_, KC = KC_; \
+// WPE: waves per SIMD the register allocation must allow (launch bound).
+// PB > 0: persistent kernel with PB workgroups per CU that loads the next
+// tile while it transforms the current one.
+#define AFFT_S4CFG(TT, NN, VV, R1_, W_, TPB_, KC_, TWF_, SUB_, WPE_, PB_)    \
+  template <>                                                                \
+  struct S4Cfg<TT, NN, VV> {                                                 \
+    static constexpr bool OK = true;                                         \
+    static constexpr int R1 = R1_, W = W_, TPB = TPB_, KC = KC_, TWF = TWF_, SUB = SUB_, \
+                         WPE = WPE_, PB = PB_;                               \
   };
-AFFT_S4CFG(float, 512, 0, 16, 16, 512, 8)
-AFFT_S4CFG(float, 512, 1, 16, 32, 1024, 8)
-AFFT_S4CFG(float, 1024, 0, 16, 8, 512, 8)
-AFFT_S4CFG(float, 1024, 1, 16, 16, 1024, 8)
-AFFT_S4CFG(float, 2048, 0, 16, 8, 1024, 16)
-AFFT_S4CFG(float, 2048, 1, 16, 4, 512, 16)
-AFFT_S4CFG(double, 512, 0, 16, 16, 512, 8)
-AFFT_S4CFG(double, 512, 1, 16, 8, 256, 16)
+AFFT_S4CFG(float, 512, 0, 16, 16, 512, 8, 0, 0, 1, 0)
+AFFT_S4CFG(float, 512, 1, 16, 32, 1024, 8, 0, 0, 1, 0)
+AFFT_S4CFG(float, 512, 2, 16, 16, 512, 8, 1, 0, 1, 0)
+AFFT_S4CFG(float, 512, 3, 16, 16, 512, 16, 0, 0, 4, 2)
+AFFT_S4CFG(float, 1024, 0, 16, 8, 512, 8, 0, 0, 1, 0)
+AFFT_S4CFG(float, 1024, 1, 16, 16, 1024, 8, 0, 0, 1, 0)
+AFFT_S4CFG(float, 1024, 2, 16, 16, 1024, 16, 0, 0, 4, 1)
+AFFT_S4CFG(float, 1024, 3, 16, 8, 512, 16, 0, 0, 4, 2)
+AFFT_S4CFG(float, 2048, 0, 16, 8, 1024, 16, 0, 0, 1, 0)
+AFFT_S4CFG(float, 2048, 1, 16, 4, 512, 16, 0, 0, 1, 0)
+AFFT_S4CFG(float, 2048, 2, 16, 8, 1024, 16, 0, 0, 4, 1)
+AFFT_S4CFG(float, 2048, 3, 16, 8, 1024, 16, 1, 0, 4, 1)
+AFFT_S4CFG(double, 512, 0, 16, 16, 512, 8, 0, 0, 1, 0)
+AFFT_S4CFG(double, 512, 1, 16, 8, 256, 16, 0, 0, 1, 0)
 #undef AFFT_S4CFG
 
+// One tile's work, shared by the one-shot and the persistent kernel.
 template <typename T, int N, int SIGN, bool BLK, int V>
-__global__ void __launch_bounds__((S4Cfg<T, N, V>::TPB))
-    sfft4_kernel(FftArgs<T> a, const cpx<T>* __restrict__ twsub) {
+struct S4Tile {
   using CF = S4Cfg<T, N, V>;
-  constexpr int R1 = CF::R1, W = CF::W, TPB = CF::TPB, KC = CF::KC;
-  constexpr int M = N / R1;
-  constexpr int D1 = W * M / TPB;   // first-level DFTs per thread
-  constexpr int RS1 = TPB / W;      // n2 step between them
-  constexpr int NT = TPB / (KC * W);  // threads per M-point line
-  constexpr int TWM = sched_twn<M, 0>();
+  static constexpr int R1 = CF::R1, W = CF::W, TPB = CF::TPB, KC = CF::KC;
+  static constexpr int M = N / R1;
+  static constexpr int D1 = W * M / TPB;      // first-level DFTs per thread
+  static constexpr int RS1 = TPB / W;         // n2 step between them
+  static constexpr int NT = TPB / (KC * W);   // threads per M-point line
+  static constexpr int SUB = CF::SUB;
+  static constexpr int TWM = sched_twn<M, SUB>();
+  static constexpr int TWF = CF::TWF;
+  static constexpr int NTW1 = TWF ? N / 2 : N;  // first-level twiddle entries in LDS
+  static_assert(!TWF || R1 == 16, "power twiddles are written for R1 = 16");
   static_assert(D1 >= 1 && D1 * TPB == W * M, "tile must split evenly");
   static_assert(NT >= 1 && NT * KC * W == TPB, "lines must split evenly");
   static_assert(R1 % KC == 0, "k1 chunks must split evenly");
-  __shared__ cpx<T> buf[KC * M * W];
-  __shared__ cpx<T> tw1[N];
-  __shared__ cpx<T> tw2[TWM > 0 ? TWM : 1];
-  const int t = threadIdx.x;
-  const int c = t % W, r0 = t / W;
-  const int64_t G = a.nlines;
-  int64_t tile = xcd_tile(blockIdx.x, gridDim.x, (unsigned)a.xgroup);
-  if (a.reverse) tile = (int64_t)gridDim.x - 1 - tile;
-  const int64_t col = tile * W + c;
-  const bool ok = col < G;
-  const int64_t ib = (col / a.C) * a.Din + (col % a.C);
-  const int64_t ob = (col / a.C) * a.Dout + (col % a.C);
 
-  // twiddle loads first, so that waiting for them does not wait for the data
-  constexpr int E1 = (N + TPB - 1) / TPB, E2 = (TWM + TPB - 1) / TPB;
-  cpx<T> tv1[E1], tv2[E2 > 0 ? E2 : 1];
+  // Issues the loads of tile `tile` (x[n2 + M n1] of column c) into v.
+  static __device__ __forceinline__ void load(cpx<T> (&v)[D1][R1], const FftArgs<T>& a,
+                                              int64_t tile, int c, int r0) {
+    const int64_t col = tile * W + c;
+    const bool ok = col < a.nlines;
+    const int64_t ib = (col / a.C) * a.Din + (col % a.C);
+    const cpx<T>* src = reinterpret_cast<const cpx<T>*>(a.in) + ib;
 #pragma unroll
-  for (int j = 0; j < E1; ++j)
-    if (t + j * TPB < N) tv1[j] = a.tw[t + j * TPB];
+    for (int d = 0; d < D1; ++d) {
+      if constexpr (BLK) {
 #pragma unroll
-  for (int j = 0; j < E2; ++j)
-    if (t + j * TPB < TWM) tv2[j] = twsub[t + j * TPB];
-  cpx<T> v[D1][R1];
-  const cpx<T>* src = reinterpret_cast<const cpx<T>*>(a.in) + ib;
+        for (int n1 = 0; n1 < R1; ++n1) {
+          const int row = r0 + d * RS1 + M * n1;
+          v[d][n1] = ok ? src[row_off(row, a.Sin, a.ishift, a.Bin)] : cpx<T>{(T)0, (T)0};
+        }
+      } else {
+        // running pointer, opaque to the compiler: the R1 row offsets are
+        // not precomputed into registers
+        const cpx<T>* p = src + (int64_t)(r0 + d * RS1) * a.Sin;
+        const int64_t step = (int64_t)M * a.Sin;
 #pragma unroll
-  for (int d = 0; d < D1; ++d) {
-    if constexpr (BLK) {
-#pragma unroll
-      for (int n1 = 0; n1 < R1; ++n1) {
-        const int row = r0 + d * RS1 + M * n1;
-        v[d][n1] = ok ? src[row_off(row, a.Sin, a.ishift, a.Bin)] : cpx<T>{(T)0, (T)0};
-      }
-    } else {
-      // running pointer, opaque to the compiler: the R1 row offsets are not
-      // precomputed into registers
-      const cpx<T>* p = src + (int64_t)(r0 + d * RS1) * a.Sin;
-      const int64_t step = (int64_t)M * a.Sin;
-#pragma unroll
-      for (int n1 = 0; n1 < R1; ++n1) {
-        v[d][n1] = ok ? *p : cpx<T>{(T)0, (T)0};
-        p += step;
-        asm volatile("" : "+v"(p));
+        for (int n1 = 0; n1 < R1; ++n1) {
+          v[d][n1] = ok ? *p : cpx<T>{(T)0, (T)0};
+          p += step;
+          asm volatile("" : "+v"(p));
+        }
       }
     }
   }
-#pragma unroll
-  for (int j = 0; j < E1; ++j)
-    if (t + j * TPB < N) tw1[t + j * TPB] = tv1[j];
-#pragma unroll
-  for (int j = 0; j < E2; ++j)
-    if (t + j * TPB < TWM) tw2[t + j * TPB] = tv2[j];
-  __syncthreads();
 
-  // first level in registers
+  // First level: R1-point DFTs and twiddles in registers.
+  static __device__ __forceinline__ void first(cpx<T> (&v)[D1][R1], int r0, const cpx<T>* tw1) {
 #pragma unroll
-  for (int d = 0; d < D1; ++d) {
-    dft<T, R1, SIGN>(v[d]);
-    const int n2 = r0 + d * RS1;
+    for (int d = 0; d < D1; ++d) {
+      dft<T, R1, SIGN>(v[d]);
+      const int n2 = r0 + d * RS1;
+      if constexpr (TWF) {
+        // w^k1 from w^1, w^2, w^4, w^8 (indices n2 * 2^b < N/2)
+        const cpx<T> p1 = tw1[n2], p2 = tw1[2 * n2], p4 = tw1[4 * n2], p8 = tw1[8 * n2];
 #pragma unroll
-    for (int k1 = 1; k1 < R1; ++k1) {
-      const cpx<T> w = tw1[n2 * k1];
-      v[d][k1] = (SIGN < 0) ? cmul(v[d][k1], w) : cmulc(v[d][k1], w);
+        for (int k1 = 1; k1 < R1; ++k1) {
+          cpx<T> w = (k1 & 1) ? p1 : (k1 & 2) ? p2 : (k1 & 4) ? p4 : p8;
+          const int lo = k1 & -k1;
+          if ((k1 & 2) && lo < 2) w = cmul(w, p2);
+          if ((k1 & 4) && lo < 4) w = cmul(w, p4);
+          if ((k1 & 8) && lo < 8) w = cmul(w, p8);
+          v[d][k1] = (SIGN < 0) ? cmul(v[d][k1], w) : cmulc(v[d][k1], w);
+        }
+      } else {
+#pragma unroll
+        for (int k1 = 1; k1 < R1; ++k1) {
+          const cpx<T> w = tw1[n2 * k1];
+          v[d][k1] = (SIGN < 0) ? cmul(v[d][k1], w) : cmulc(v[d][k1], w);
+        }
+      }
     }
   }
 
-  // second level in LDS, KC values of k1 at a time
-  const int q = r0 % NT, kl = r0 / NT;
-#pragma unroll
-  for (int kc0 = 0; kc0 < R1; kc0 += KC) {
+  // Values k1 = kc0 .. kc0+KC-1 of the first level into the tile buffer.
+  static __device__ __forceinline__ void put(const cpx<T> (&v)[D1][R1], int kc0, int c, int r0,
+                                             cpx<T>* buf) {
 #pragma unroll
     for (int d = 0; d < D1; ++d)
 #pragma unroll
       for (int kk = 0; kk < KC; ++kk) buf[(kk * M + r0 + d * RS1) * W + c] = v[d][kc0 + kk];
-    __syncthreads();
+  }
+
+  // Second level of one chunk (M-point DFTs over n2 in LDS) and the stores
+  // of X[k1 + R1 k2], k1 in the chunk.  Caller has put the chunk and
+  // synchronised; ends after a barrier (buffer free again).
+  static __device__ __forceinline__ void second(const FftArgs<T>& a, int64_t tile, int kc0, int c,
+                                                int r0, cpx<T>* buf, const cpx<T>* tw2) {
+    const int64_t col = tile * W + c;
+    const bool ok = col < a.nlines;
+    const int64_t ob = (col / a.C) * a.Dout + (col % a.C);
+    const int q = r0 % NT, kl = r0 / NT;
     LineAcc<T, W> acc{buf + kl * M * W + c};
-    static_fft<T, M, NT, SIGN, W, 0, 1>(acc, q, tw2, buf, 0);
+    static_fft<T, M, NT, SIGN, W, SUB, 1>(acc, q, tw2, buf, 0);
     if (ok) {
       cpx<T>* dst = reinterpret_cast<cpx<T>*>(a.out) + ob;
+      if constexpr (BLK) {
 #pragma unroll
-      for (int kk = 0; kk < KC; ++kk)
+        for (int kk = 0; kk < KC; ++kk)
 #pragma unroll
-        for (int k2 = r0; k2 < M; k2 += RS1) {
-          const int row = kc0 + kk + R1 * k2;
-          const int64_t off = BLK ? row_off(row, a.Sout, a.oshift, a.Bout) : (int64_t)row * a.Sout;
-          dst[off] = buf[(kk * M + k2) * W + c];
+          for (int k2 = r0; k2 < M; k2 += RS1) {
+            const int row = kc0 + kk + R1 * k2;
+            dst[row_off(row, a.Sout, a.oshift, a.Bout)] = buf[(kk * M + k2) * W + c];
+          }
+      } else {
+        // running pointers (rows kc0 + kk + R1 * k2), opaque to the
+        // compiler so that the row addresses are not all kept in registers
+        cpx<T>* p0 = dst + (int64_t)(kc0 + R1 * r0) * a.Sout;
+        const int64_t step = (int64_t)R1 * RS1 * a.Sout;
+#pragma unroll
+        for (int kk = 0; kk < KC; ++kk) {
+          cpx<T>* p = p0;
+#pragma unroll
+          for (int k2 = r0; k2 < M; k2 += RS1) {
+            *p = buf[(kk * M + k2) * W + c];
+            p += step;
+            asm volatile("" : "+v"(p));
+          }
+          p0 += a.Sout;
+          asm volatile("" : "+v"(p0));
         }
+      }
     }
     __syncthreads();
+  }
+
+  // The whole tile: first level, then the chunks of the second level.
+  static __device__ __forceinline__ void run(cpx<T> (&v)[D1][R1], const FftArgs<T>& a,
+                                             int64_t tile, int c, int r0, cpx<T>* buf,
+                                             const cpx<T>* tw1, const cpx<T>* tw2) {
+    first(v, r0, tw1);
+#pragma unroll
+    for (int kc0 = 0; kc0 < R1; kc0 += KC) {
+      put(v, kc0, c, r0, buf);
+      __syncthreads();
+      second(a, tile, kc0, c, r0, buf, tw2);
+    }
+  }
+
+  // Twiddles into LDS: the loads are issued first (tv1/tv2), the LDS writes
+  // after the caller's data loads so that all loads are in flight together.
+  static constexpr int E1 = (NTW1 + TPB - 1) / TPB, E2 = (TWM + TPB - 1) / TPB > 0 ? (TWM + TPB - 1) / TPB : 1;
+  static __device__ __forceinline__ void tw_load(cpx<T> (&tv1)[E1], cpx<T> (&tv2)[E2],
+                                                 const FftArgs<T>& a, const cpx<T>* twsub, int t) {
+#pragma unroll
+    for (int j = 0; j < E1; ++j)
+      if (t + j * TPB < NTW1) tv1[j] = a.tw[t + j * TPB];
+#pragma unroll
+    for (int j = 0; j < E2; ++j)
+      if (t + j * TPB < TWM) tv2[j] = twsub[t + j * TPB];
+  }
+  static __device__ __forceinline__ void tw_store(const cpx<T> (&tv1)[E1], const cpx<T> (&tv2)[E2],
+                                                  cpx<T>* tw1, cpx<T>* tw2, int t) {
+#pragma unroll
+    for (int j = 0; j < E1; ++j)
+      if (t + j * TPB < NTW1) tw1[t + j * TPB] = tv1[j];
+#pragma unroll
+    for (int j = 0; j < E2; ++j)
+      if (t + j * TPB < TWM) tw2[t + j * TPB] = tv2[j];
+  }
+
+  // Virtual block vb of a launch of nb blocks -> tile (XCD chunk map, order).
+  static __device__ __forceinline__ int64_t tile_of(int64_t vb, int64_t nb, const FftArgs<T>& a) {
+    int64_t tile = xcd_tile((unsigned)vb, (unsigned)nb, (unsigned)a.xgroup);
+    return a.reverse ? nb - 1 - tile : tile;
+  }
+};
+
+template <typename T, int N, int SIGN, bool BLK, int V>
+__global__ void __launch_bounds__((S4Cfg<T, N, V>::TPB), (S4Cfg<T, N, V>::WPE))
+    sfft4_kernel(FftArgs<T> a, const cpx<T>* __restrict__ twsub) {
+  using K = S4Tile<T, N, SIGN, BLK, V>;
+  __shared__ cpx<T> buf[K::KC * K::M * K::W];
+  __shared__ cpx<T> tw1[K::NTW1];
+  __shared__ cpx<T> tw2[K::TWM > 0 ? K::TWM : 1];
+  const int t = threadIdx.x;
+  const int c = t % K::W, r0 = t / K::W;
+  cpx<T> tv1[K::E1], tv2[K::E2];
+  K::tw_load(tv1, tv2, a, twsub, t);
+  cpx<T> v[K::D1][K::R1];
+  if constexpr (S4Cfg<T, N, V>::PB == 0) {
+    const int64_t tile = K::tile_of(blockIdx.x, gridDim.x, a);
+    K::load(v, a, tile, c, r0);
+    K::tw_store(tv1, tv2, tw1, tw2, t);
+    __syncthreads();
+    K::run(v, a, tile, c, r0, buf, tw1, tw2);
+  } else {
+    // Persistent: block b takes the virtual blocks b, b + grid, ... of a
+    // launch of one block per tile (grid a multiple of 8, so each keeps its
+    // XCD).  The whole first level goes to LDS in one chunk, after which
+    // the registers take the next tile's loads while the second level and
+    // the stores of the current tile run.
+    static_assert(K::KC == K::R1, "persistent tiles need the whole first level in LDS");
+    const int64_t ntiles = (a.nlines + K::W - 1) / K::W;
+    int64_t vb = blockIdx.x;
+    int64_t tile = K::tile_of(vb, ntiles, a);
+    K::load(v, a, tile, c, r0);
+    K::tw_store(tv1, tv2, tw1, tw2, t);
+    __syncthreads();
+    for (;;) {
+      K::first(v, r0, tw1);
+      K::put(v, 0, c, r0, buf);
+      __syncthreads();
+      const int64_t nvb = vb + gridDim.x;
+      const bool more = nvb < ntiles;
+      const int64_t ntile = more ? K::tile_of(nvb, ntiles, a) : 0;
+      if (more) K::load(v, a, ntile, c, r0);
+      K::second(a, tile, 0, c, r0, buf, tw2);
+      if (!more) break;
+      vb = nvb;
+      tile = ntile;
+    }
   }
 }
 

@@ -1,6 +1,9 @@
 // fft_launch.hpp -- host-side selection of an FFT kernel instance and its
 // launch shape.  Included once per precision (fft_f32.hip, fft_f64.hip).
 #pragma once
+#include <utility>
+#include <vector>
+
 #include "fft_kernels.hpp"
 
 namespace afft {
@@ -15,6 +18,7 @@ struct AxisTab {
   const cpx<T>* tw_full = nullptr;    // exp(-2 pi i m/n), m < n
   const cpx<T>* tw_split = nullptr;   // per-stage table of the split schedule
   const cpx<T>* tw_sub = nullptr;     // per-stage table of length n/16 (two-level tiles)
+  const cpx<T>* tw_subw = nullptr;    // the same for the SCHED_WIDE schedule
 };
 
 // Lengths with a matrix-core (split-precision) schedule.
@@ -129,9 +133,9 @@ inline bool wide_tiles(int n) {
 
 // Two-level strided C2C tiles (sfft4_kernel): on by default where they were
 // measured faster on MI355X (FP32 512, 1024, 2048; FP64 512 is faster with
-// the one-level tiles); the masks ACCFFT_AMD_S4 (bit log2(n): use them) and
-// ACCFFT_AMD_S4ALT (bit: use the alternative configuration) override, for
-// measurement.
+// the one-level tiles).  For measurement, ACCFFT_AMD_S4 (bit log2(n): use
+// them) overrides that choice and ACCFFT_AMD_S4V ("n:v,n:v", e.g.
+// "2048:2,1024:3") picks the configuration variant of a length.
 inline unsigned env_mask(const char* name, unsigned dflt) {
   const char* e = getenv(name);
   return e ? (unsigned)strtoul(e, nullptr, 0) : dflt;
@@ -147,9 +151,43 @@ inline bool four_step(int n) {
       env_mask("ACCFFT_AMD_S4", sizeof(T) == 4 ? (1u << 9) | (1u << 10) | (1u << 11) : 0u);
   return (mask >> ilog2i(n)) & 1u;
 }
-inline bool four_step_alt(int n) {
-  static const unsigned mask = env_mask("ACCFFT_AMD_S4ALT", 0u);
-  return (mask >> ilog2i(n)) & 1u;
+// Variant of the two-level configuration for length n (S4Cfg<T, n, V>);
+// blk: rows in exchange blocks (the slab plan's y pass).  Default: the
+// persistent 2048 tiles for plain rows (measured 8 % faster on the 2048
+// x passes of the 256 x 2048 x 2048 slab), variant 0 otherwise.
+inline int four_step_variant(int n, bool blk) {
+  static const std::vector<std::pair<int, int>> tab = [] {
+    std::vector<std::pair<int, int>> t;
+    const char* e = getenv("ACCFFT_AMD_S4V");
+    while (e && *e) {
+      char* end = nullptr;
+      const long len = strtol(e, &end, 10);
+      if (end == e || *end != ':') break;
+      e = end + 1;
+      const long v = strtol(e, &end, 10);
+      if (end == e) break;
+      t.emplace_back((int)len, (int)v);
+      e = *end == ',' ? end + 1 : end;
+    }
+    return t;
+  }();
+  for (const auto& kv : tab)
+    if (kv.first == n) return kv.second;
+  return (n == 2048 && !blk) ? 2 : 0;
+}
+
+// Compute units of the current device (read once per process).
+inline int device_cus() {
+  static const int cus = [] {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) {
+      (void)hipGetLastError();
+      return 256;
+    }
+    return n;
+  }();
+  return cus;
 }
 
 template <typename T, int N, int V>
@@ -161,9 +199,16 @@ hipError_t launch_s4v(int sign, FftArgs<T>& a, const AxisTab<T>& tab, hipStream_
     const int64_t tiles = (a.nlines + CF::W - 1) / CF::W;
     a.tw = tab.tw_full;
     a.lpw = CF::W;
-    dim3 g((unsigned)tiles), b(CF::TPB);
+    int64_t grid = tiles;
+    if constexpr (CF::PB > 0) {
+      // persistent: PB workgroups per CU, a multiple of 8 (one per XCD)
+      const int64_t cap = (int64_t)device_cus() * CF::PB;
+      if (grid > cap) grid = cap >= 8 ? cap / 8 * 8 : cap;
+    }
+    dim3 g((unsigned)grid), b(CF::TPB);
     const bool blk = a.ishift != 0 || a.oshift != 0;
-#define AFFT_S4(S, B) hipLaunchKernelGGL((sfft4_kernel<T, N, S, B, V>), g, b, 0, st, a, tab.tw_sub)
+    const cpx<T>* twsub = CF::SUB == SCHED_WIDE ? tab.tw_subw : tab.tw_sub;
+#define AFFT_S4(S, B) hipLaunchKernelGGL((sfft4_kernel<T, N, S, B, V>), g, b, 0, st, a, twsub)
     if (blk) {
       if (sign < 0) AFFT_S4(-1, true);
       else AFFT_S4(+1, true);
@@ -178,8 +223,12 @@ hipError_t launch_s4v(int sign, FftArgs<T>& a, const AxisTab<T>& tab, hipStream_
 
 template <typename T, int N>
 hipError_t launch_s4(int sign, FftArgs<T>& a, const AxisTab<T>& tab, hipStream_t st) {
-  return four_step_alt(N) ? launch_s4v<T, N, 1>(sign, a, tab, st)
-                          : launch_s4v<T, N, 0>(sign, a, tab, st);
+  switch (four_step_variant(N, a.ishift != 0 || a.oshift != 0)) {
+    case 1: return launch_s4v<T, N, 1>(sign, a, tab, st);
+    case 2: return launch_s4v<T, N, 2>(sign, a, tab, st);
+    case 3: return launch_s4v<T, N, 3>(sign, a, tab, st);
+    default: return launch_s4v<T, N, 0>(sign, a, tab, st);
+  }
 }
 
 // Launch one axis pass.  strided = false: contiguous lines (z layout).

@@ -246,18 +246,18 @@ __device__ __forceinline__ void static_fft(const LineAcc<T, ES>& acc, int q,
                                            const cpx<T>* __restrict__ tw, cpx<T>* tile, int LS) {
   using S = SchedOf<N, SPLIT>;
   constexpr int R0 = S::r(0), R1 = S::r(1), R2 = S::r(2);
-  if constexpr (SPLIT && R0 == 16)
+  if constexpr (split_sched(SPLIT) && R0 == 16)
     mstage<N, 1, SIGN, 0, ES, LINES, SPLIT>(tile, LS, tw);
   else
     sstage<T, N, R0, 1, NT, SIGN, 0, ES>(acc, q, tw);
   if constexpr (S::S > 1) {
-    if constexpr (SPLIT && R1 == 16)
+    if constexpr (split_sched(SPLIT) && R1 == 16)
       mstage<N, sched_ns<N, SPLIT>(1), SIGN, sched_twoff<N, SPLIT>(1), ES, LINES, SPLIT>(tile, LS, tw);
     else
       sstage<T, N, R1, sched_ns<N, SPLIT>(1), NT, SIGN, sched_twoff<N, SPLIT>(1), ES>(acc, q, tw);
   }
   if constexpr (S::S > 2) {
-    if constexpr (SPLIT && R2 == 16)
+    if constexpr (split_sched(SPLIT) && R2 == 16)
       mstage<N, sched_ns<N, SPLIT>(2), SIGN, sched_twoff<N, SPLIT>(2), ES, LINES, SPLIT>(tile, LS, tw);
     else
       sstage<T, N, R2, sched_ns<N, SPLIT>(2), NT, SIGN, sched_twoff<N, SPLIT>(2), ES>(acc, q, tw);

@@ -99,11 +99,16 @@ inline cpx<T> host_root(long m, long n) {
   return {(T)cosl(a), (T)(-sinl(a))};
 }
 
+// split: schedule id (0, SCHED_WIDE or a split-precision kind)
 inline int static_sched(int n, int R[3], int split = 0) {
 #define AFFT_SCHED(NN)                                              \
   case NN:                                                          \
     for (int i = 0; i < 3; ++i) R[i] = SchedOf<NN, 0>::r(i);        \
-    if (split && SchedSplit<NN>::S > 0) {                           \
+    if (split == SCHED_WIDE && SchedWide<NN>::S > 0) {              \
+      for (int i = 0; i < 3; ++i) R[i] = SchedWide<NN>::R[i];       \
+      return SchedWide<NN>::S;                                      \
+    }                                                               \
+    if (split_sched(split) && SchedSplit<NN>::S > 0) {              \
       for (int i = 0; i < 3; ++i) R[i] = SchedSplit<NN>::R[i];      \
       return SchedSplit<NN>::S;                                     \
     }                                                               \
