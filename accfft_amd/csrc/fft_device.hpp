@@ -209,13 +209,16 @@ template <> struct Sched<1024> { static constexpr int S = 3; static constexpr in
 template <> struct Sched<2048> { static constexpr int S = 3; static constexpr int R[3] = {16, 16, 8}; };
 template <> struct Sched<4096> { static constexpr int S = 3; static constexpr int R[3] = {16, 16, 16}; };
 
-// Schedules used with the split-precision matrix-core stages: as many
-// radix-16 stages as possible, the remainder first.
+// Schedules used with the split-precision matrix-core stages: a radix-16
+// first stage (on the matrix cores, no twiddles), then the smaller radices
+// of the VALU schedule; 2048 and 4096 need more than one radix-16 stage.
+// Each matrix-core stage costs more VALU work (operand splitting) than a
+// VALU radix-16 stage, so the schedules keep them few.
 template <int N> struct SchedSplit { static constexpr int S = 0; static constexpr int R[3] = {0, 0, 0}; };
-template <> struct SchedSplit<256> { static constexpr int S = 2; static constexpr int R[3] = {16, 16, 0}; };
-template <> struct SchedSplit<512> { static constexpr int S = 3; static constexpr int R[3] = {2, 16, 16}; };
-template <> struct SchedSplit<1024> { static constexpr int S = 3; static constexpr int R[3] = {4, 16, 16}; };
-template <> struct SchedSplit<2048> { static constexpr int S = 3; static constexpr int R[3] = {8, 16, 16}; };
+template <> struct SchedSplit<256> { static constexpr int S = 3; static constexpr int R[3] = {16, 4, 4}; };
+template <> struct SchedSplit<512> { static constexpr int S = 3; static constexpr int R[3] = {16, 8, 4}; };
+template <> struct SchedSplit<1024> { static constexpr int S = 3; static constexpr int R[3] = {16, 8, 8}; };
+template <> struct SchedSplit<2048> { static constexpr int S = 3; static constexpr int R[3] = {16, 16, 8}; };
 template <> struct SchedSplit<4096> { static constexpr int S = 3; static constexpr int R[3] = {16, 16, 16}; };
 
 // Schedules with more, smaller radices (more butterflies per stage, so more

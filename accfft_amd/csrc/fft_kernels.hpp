@@ -619,7 +619,9 @@ AFFT_S4CFG(double, 512, 1, 16, 8, 256, 16, 0, 0, 1, 0)
 #undef AFFT_S4CFG
 
 // One tile's work, shared by the one-shot and the persistent kernel.
-template <typename T, int N, int SIGN, bool BLK, int V>
+// SPLIT != 0 (FP32): the first level's radix-16 DFTs run on the matrix cores
+// with split operands (fft_mfma.hpp); the second level stays on the VALU.
+template <typename T, int N, int SIGN, bool BLK, int V, int SPLIT = 0>
 struct S4Tile {
   using CF = S4Cfg<T, N, V>;
   static constexpr int R1 = CF::R1, W = CF::W, TPB = CF::TPB, KC = CF::KC;
@@ -635,6 +637,9 @@ struct S4Tile {
   static_assert(D1 >= 1 && D1 * TPB == W * M, "tile must split evenly");
   static_assert(NT >= 1 && NT * KC * W == TPB, "lines must split evenly");
   static_assert(R1 % KC == 0, "k1 chunks must split evenly");
+  static_assert(!SPLIT || (R1 == 16 && TWF == 0 && 32 % W == 0 && TPB % 64 == 0 &&
+                           (KC == 8 || KC == 16)),
+                "matrix-core first level: R1 = 16, plain twiddles, lanes l and l^32 in one column");
 
   // Issues the loads of tile `tile` (x[n2 + M n1] of column c) into v.
   static __device__ __forceinline__ void load(cpx<T> (&v)[D1][R1], const FftArgs<T>& a,
@@ -667,7 +672,48 @@ struct S4Tile {
   }
 
   // First level: R1-point DFTs and twiddles in registers.
-  static __device__ __forceinline__ void first(cpx<T> (&v)[D1][R1], int r0, const cpx<T>* tw1) {
+  //
+  // Matrix-core form: lanes l and l^32 of a wavefront hold butterflies A
+  // (the lane with bit 5 clear) and B.  They swap halves (one
+  // v_permlane32_swap per register pair, no LDS traffic) so that every
+  // lane holds inputs 8h..8h+7 of A in v[0..7] and of B in v[8..15]
+  // (h = bit 5), two 32-butterfly products give each lane outputs
+  // q = dft16_q(j, h) of A (v[j]) and B (v[8 + j]), j < 8, and put() writes
+  // them to their k1 = q rows directly (no swap back).
+  static __device__ __forceinline__ void first(cpx<T> (&v)[D1][R1], int t, const cpx<T>* tw1) {
+    if constexpr (SPLIT != 0) {
+      using SF = SplitFmt<SPLIT>;
+      const int h = (t >> 5) & 1;
+      typename SF::V A0[SF::P], A1[SF::P];
+      dft16_operands<SPLIT, SIGN>(A0, A1);
+#pragma unroll
+      for (int d = 0; d < D1; ++d) {
+        cpx<float> x0[8], x1[8], y0[8], y1[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          // v_permlane32_swap(x, y) swaps lanes 32..63 of x with lanes 0..31
+          // of y: x becomes (A_i | A_{8+i}), y becomes (B_i | B_{8+i})
+          const auto re = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[d][i].x),
+                                                           __float_as_uint(v[d][8 + i].x), false, false);
+          const auto im = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[d][i].y),
+                                                           __float_as_uint(v[d][8 + i].y), false, false);
+          x0[i] = {__uint_as_float(re[0]), __uint_as_float(im[0])};
+          x1[i] = {__uint_as_float(re[1]), __uint_as_float(im[1])};
+        }
+        dft16_mfma<SPLIT>(x0, A0, A1, y0);
+        dft16_mfma<SPLIT>(x1, A0, A1, y1);
+        const int n2a = (t & ~32) / W + d * RS1, n2b = (t | 32) / W + d * RS1;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const int q = 8 * (j >> 2) + 4 * h + (j & 3);
+          const cpx<T> wa = tw1[n2a * q], wb = tw1[n2b * q];
+          v[d][j] = (SIGN < 0) ? cmul(y0[j], wa) : cmulc(y0[j], wa);
+          v[d][8 + j] = (SIGN < 0) ? cmul(y1[j], wb) : cmulc(y1[j], wb);
+        }
+      }
+      return;
+    }
+    const int r0 = t / W;
 #pragma unroll
     for (int d = 0; d < D1; ++d) {
       dft<T, R1, SIGN>(v[d]);
@@ -697,6 +743,23 @@ struct S4Tile {
   // Values k1 = kc0 .. kc0+KC-1 of the first level into the tile buffer.
   static __device__ __forceinline__ void put(const cpx<T> (&v)[D1][R1], int kc0, int c, int r0,
                                              cpx<T>* buf) {
+    if constexpr (SPLIT != 0) {
+      // v[d][j]: output q of butterfly A (j < 8) or B; chunks of 8 hold the
+      // j with (j & 7) < 4 (q < 8) first
+      const int t = threadIdx.x, h = (t >> 5) & 1;
+#pragma unroll
+      for (int d = 0; d < D1; ++d) {
+        const int n2a = (t & ~32) / W + d * RS1, n2b = (t | 32) / W + d * RS1;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) {
+          const int jj = j & 7;
+          if (KC == 8 && ((jj < 4) != (kc0 == 0))) continue;
+          const int q = 8 * (jj >> 2) + 4 * h + (jj & 3);
+          buf[((q - kc0) * M + (j < 8 ? n2a : n2b)) * W + c] = v[d][j];
+        }
+      }
+      return;
+    }
 #pragma unroll
     for (int d = 0; d < D1; ++d)
 #pragma unroll
@@ -750,7 +813,7 @@ struct S4Tile {
   static __device__ __forceinline__ void run(cpx<T> (&v)[D1][R1], const FftArgs<T>& a,
                                              int64_t tile, int c, int r0, cpx<T>* buf,
                                              const cpx<T>* tw1, const cpx<T>* tw2) {
-    first(v, r0, tw1);
+    first(v, (int)threadIdx.x, tw1);
 #pragma unroll
     for (int kc0 = 0; kc0 < R1; kc0 += KC) {
       put(v, kc0, c, r0, buf);
@@ -788,10 +851,10 @@ struct S4Tile {
   }
 };
 
-template <typename T, int N, int SIGN, bool BLK, int V>
+template <typename T, int N, int SIGN, bool BLK, int V, int SPLIT = 0>
 __global__ void __launch_bounds__((S4Cfg<T, N, V>::TPB), (S4Cfg<T, N, V>::WPE))
     sfft4_kernel(FftArgs<T> a, const cpx<T>* __restrict__ twsub) {
-  using K = S4Tile<T, N, SIGN, BLK, V>;
+  using K = S4Tile<T, N, SIGN, BLK, V, SPLIT>;
   __shared__ cpx<T> buf[K::KC * K::M * K::W];
   __shared__ cpx<T> tw1[K::NTW1];
   __shared__ cpx<T> tw2[K::TWM > 0 ? K::TWM : 1];
@@ -820,7 +883,7 @@ __global__ void __launch_bounds__((S4Cfg<T, N, V>::TPB), (S4Cfg<T, N, V>::WPE))
     K::tw_store(tv1, tv2, tw1, tw2, t);
     __syncthreads();
     for (;;) {
-      K::first(v, r0, tw1);
+      K::first(v, t, tw1);
       K::put(v, 0, c, r0, buf);
       __syncthreads();
       const int64_t nvb = vb + gridDim.x;

@@ -190,7 +190,7 @@ inline int device_cus() {
   return cus;
 }
 
-template <typename T, int N, int V>
+template <typename T, int N, int V, int SPLIT = 0>
 hipError_t launch_s4v(int sign, FftArgs<T>& a, const AxisTab<T>& tab, hipStream_t st) {
   if constexpr (!S4Cfg<T, N, V>::OK) {
     return hipErrorNotSupported;
@@ -208,7 +208,7 @@ hipError_t launch_s4v(int sign, FftArgs<T>& a, const AxisTab<T>& tab, hipStream_
     dim3 g((unsigned)grid), b(CF::TPB);
     const bool blk = a.ishift != 0 || a.oshift != 0;
     const cpx<T>* twsub = CF::SUB == SCHED_WIDE ? tab.tw_subw : tab.tw_sub;
-#define AFFT_S4(S, B) hipLaunchKernelGGL((sfft4_kernel<T, N, S, B, V>), g, b, 0, st, a, twsub)
+#define AFFT_S4(S, B) hipLaunchKernelGGL((sfft4_kernel<T, N, S, B, V, SPLIT>), g, b, 0, st, a, twsub)
     if (blk) {
       if (sign < 0) AFFT_S4(-1, true);
       else AFFT_S4(+1, true);

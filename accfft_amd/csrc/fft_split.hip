@@ -30,6 +30,34 @@ static hipError_t launch_split_n(bool strided, int mode, int sign, FftArgs<float
   return hipGetLastError();
 }
 
+// Two-level strided tiles whose first radix-16 level runs on the matrix
+// cores (one-shot configurations that keep lanes l and l^32 in one column).
+template <int N, int V>
+constexpr bool split_s4_ok() {
+  using CF = S4Cfg<float, N, V>;
+  return CF::OK && CF::PB == 0 && CF::R1 == 16 && CF::TWF == 0 && 32 % CF::W == 0 && CF::TPB % 64 == 0 &&
+         (CF::KC == 8 || CF::KC == 16);
+}
+
+template <int N, int SPLIT>
+static hipError_t launch_split_s4(int sign, FftArgs<float>& a, const AxisTab<float>& tab,
+                                  hipStream_t st) {
+  a.sc = tab.sc;
+  a.sc.n = tab.n;
+  switch (four_step_variant(N, a.ishift != 0 || a.oshift != 0)) {
+#define AFFT_SV(VV) \
+  case VV:          \
+    if constexpr (split_s4_ok<N, VV>()) return launch_s4v<float, N, VV, SPLIT>(sign, a, tab, st); \
+    break;
+    AFFT_SV(1)
+    AFFT_SV(2)
+    AFFT_SV(3)
+#undef AFFT_SV
+    default: break;
+  }
+  return launch_s4v<float, N, 0, SPLIT>(sign, a, tab, st);
+}
+
 template <int SPLIT>
 static hipError_t launch_split_k(bool strided, int mode, int sign, FftArgs<float>& a,
                                  int64_t units, int n, hipStream_t st) {
@@ -47,6 +75,20 @@ hipError_t launch_fft_split(bool strided, int mode, int sign, FftArgs<float> a,
                             const AxisTab<float>& tab, int split, hipStream_t st) {
   const int64_t units = pair_mode(mode) ? (a.nlines + 1) / 2 : a.nlines;
   if (units <= 0) return hipSuccess;
+  if (strided && mode == M_C2C && tab.tw_sub && four_step<float>(tab.n) &&
+      (tab.n == 512 || tab.n == 1024 || tab.n == 2048)) {
+#define AFFT_S4S(NN)                                                                    \
+  if (tab.n == NN) switch (split) {                                                     \
+      case SPLIT_BF16: return launch_split_s4<NN, SPLIT_BF16>(sign, a, tab, st);        \
+      case SPLIT_BF16X3: return launch_split_s4<NN, SPLIT_BF16X3>(sign, a, tab, st);    \
+      case SPLIT_FP16: return launch_split_s4<NN, SPLIT_FP16>(sign, a, tab, st);        \
+      default: return hipErrorInvalidValue;                                             \
+    }
+    AFFT_S4S(512)
+    AFFT_S4S(1024)
+    AFFT_S4S(2048)
+#undef AFFT_S4S
+  }
   a.sc = tab.sc;
   a.sc.n = tab.n;
   a.tw = tab.tw_split;
