@@ -30,7 +30,8 @@ PRESERVE_INPUT = 0x200
 F64, F32 = 0, 1
 OK = 0
 
-GRAD_X, GRAD_Y, GRAD_Z, LAPLACE, BIHARMONIC, INV_LAPLACE, INV_BIHARMONIC = range(7)
+(GRAD_X, GRAD_Y, GRAD_Z, LAPLACE, BIHARMONIC, INV_LAPLACE, INV_BIHARMONIC, POISSON_DISCRETE,
+ POISSON_CONTINUOUS) = range(9)
 
 _lib = None
 _i3 = ctypes.c_int * 3
@@ -83,6 +84,8 @@ def lib() -> ctypes.CDLL:
                "accfft_amd_inv_biharmonic"):
         getattr(L, nm).argtypes = [_vp, _vp, _vp, dp]
     L.accfft_amd_divergence.argtypes = [_vp, _vp, _vp, _vp, _vp, dp, ctypes.c_int]
+    L.accfft_amd_poisson_solve.argtypes = [_vp, ctypes.c_int, _vp, _vp, dp]
+    L.accfft_amd_poisson_filter.argtypes = [ctypes.c_int, _vp, _i3, _i3, _i3, ctypes.c_int, _vp]
     L.accfft_amd_mult_wave_number.argtypes = [ctypes.c_int, ctypes.c_int, _vp, _vp, _i3, _i3,
                                               _i3, ctypes.c_uint, _vp]
     L.accfft_amd_axpy.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_double, _vp, _vp, _vp]
@@ -93,6 +96,12 @@ def lib() -> ctypes.CDLL:
                                    _vp, _vp, ctypes.c_int, _vp]
     L.accfft_amd_local_transpose_inplace.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64,
                                                      _vp, ctypes.c_int, _vp]
+    L.accfft_amd_pencil_pack.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int64,
+                                         i64p, _vp, _vp, ctypes.c_int64, ctypes.c_int64, i64p,
+                                         ctypes.c_int, _vp]
+    L.accfft_amd_pencil_unpack.argtypes = [ctypes.c_int, ctypes.c_int, i64p, i64p, _vp,
+                                           ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _vp,
+                                           ctypes.c_int, _vp]
     L.accfft_amd_adjoint_r2c.argtypes = [_vp, _vp, _vp, dp]
     L.accfft_amd_adjoint_c2r.argtypes = [_vp, _vp, _vp, dp]
     L.accfft_amd_adjoint_grad.argtypes = [_vp, _vp, _vp, _vp, _vp, dp]
@@ -284,6 +293,15 @@ class Plan:
 
     def inv_biharmonic(self, A, out, timer=None):
         self._spectral("accfft_amd_inv_biharmonic", out, A, timer)
+
+    def poisson_solve(self, rho, phi, method: int = 0, timer=None):
+        """The step4 FFT Poisson solve (steps/step4/step4_gpu.cpp:520-640) with
+        the Fourier filter (steps/step4/kernels.cu:24-110) fused into the x
+        pass: phi = zero-mean solution of laplace(phi) = rho on the unit box."""
+        arr, tp = self._timer(timer)
+        _check(lib().accfft_amd_poisson_solve(self._bind(), int(method), _ptr(phi), _ptr(rho), tp),
+               "poisson_solve")
+        self._untimer(timer, arr)
 
     def divergence(self, Ax, Ay, Az, out, timer=None, slow=False):
         arr, tp = self._timer(timer)
@@ -481,6 +499,14 @@ def mult_wave_number(kind: int, wA, A, n, osize, ostart, scale_xyz: int = 7):
            "mult_wave_number")
 
 
+def poisson_fourier_filter_gpu(data_hat, N, isize, istart, method: int = 0):
+    """poisson_fourier_filter_gpu(data_hat, N, isize, istart, methodNb),
+    steps/step4/kernels.cu:113-146, in place on a half spectrum block."""
+    _check(lib().accfft_amd_poisson_filter(_fp_of(data_hat.dtype), _ptr(data_hat), _i3(*N),
+                                           _i3(*isize), _i3(*istart), int(method), _stream()),
+           "poisson_filter")
+
+
 def axpy(alpha: float, x, y):
     """daxpy_gpu_c(f), src/operators_cuda.cu:470-474: y += alpha * x."""
     _check(lib().accfft_amd_axpy(_fp_of(x.dtype), x.numel(), alpha, _ptr(x), _ptr(y), _stream()),
@@ -511,3 +537,26 @@ def local_transpose_inplace(r, c, t, A):
     """local_transpose_cuda(r,c,t,A), src/transpose_cuda.cu:227-239."""
     _check(lib().accfft_amd_local_transpose_inplace(r, c, t, _ptr(A), A.element_size(),
                                                     _stream()), "local_transpose_inplace")
+
+
+def _i64arr(v):
+    v = [int(x) for x in v]
+    return (ctypes.c_int64 * max(len(v), 1))(*v)
+
+
+def memcpy_v1_h1(nprocs_1, howmany, local_n0, n_tuples, local_n1_proc, send_recv, data, idist,
+                 N1, local_1_start_proc):
+    """Pencil pack, src/transpose_cuda.cu:97-143 (see accfft_amd_pencil_pack)."""
+    _check(lib().accfft_amd_pencil_pack(nprocs_1, howmany, local_n0, n_tuples,
+                                        _i64arr(local_n1_proc), _ptr(send_recv), _ptr(data), idist,
+                                        N1, _i64arr(local_1_start_proc), data.element_size(),
+                                        _stream()), "pencil_pack")
+
+
+def memcpy_v1_h2(nprocs_0, howmany, local_0_start_proc, local_n0_proc, data, odist, local_n1,
+                 n_tuples, send_recv):
+    """Pencil unpack, src/transpose_cuda.cu:145-197 (see accfft_amd_pencil_unpack)."""
+    _check(lib().accfft_amd_pencil_unpack(nprocs_0, howmany, _i64arr(local_0_start_proc),
+                                          _i64arr(local_n0_proc), _ptr(data), odist, local_n1,
+                                          n_tuples, _ptr(send_recv), data.element_size(),
+                                          _stream()), "pencil_unpack")

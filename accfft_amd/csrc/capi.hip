@@ -410,6 +410,12 @@ int accfft_amd_inv_laplace(accfft_amd_plan* p, void* out, const void* A, double*
 int accfft_amd_inv_biharmonic(accfft_amd_plan* p, void* out, const void* A, double* timer) {
   return spectral(p, afft::SYM_INV_BIHARMONIC, out, A, timer);
 }
+int accfft_amd_poisson_solve(accfft_amd_plan* p, int method, void* phi, const void* rho,
+                             double* timer) {
+  if (method != 0 && method != 1) return ACCFFT_AMD_ERR_INVALID;
+  return spectral(p, method == 0 ? afft::SYM_POISSON_DISCRETE : afft::SYM_POISSON_CONTINUOUS,
+                  phi, rho, timer);
+}
 
 int accfft_amd_divergence(accfft_amd_plan* p, void* divA, const void* Ax, const void* Ay,
                           const void* Az, double* timer, int slow) {
@@ -425,7 +431,7 @@ int accfft_amd_mult_wave_number(int kind, int fp, void* wA, const void* A, const
                                 const int osize[3], const int ostart[3], unsigned scale_xyz,
                                 void* stream) {
   if (!wA || !A || !n || !osize || !ostart) return ACCFFT_AMD_ERR_NULL;
-  if (kind < 0 || kind > 6) return ACCFFT_AMD_ERR_INVALID;
+  if (kind < 0 || kind > 8) return ACCFFT_AMD_ERR_INVALID;
   if (fp != ACCFFT_AMD_F32 && fp != ACCFFT_AMD_F64) return ACCFFT_AMD_ERR_INVALID;
   for (int i = 0; i < 3; ++i)
     if (n[i] <= 0 || osize[i] < 0) return ACCFFT_AMD_ERR_INVALID;
@@ -439,6 +445,14 @@ int accfft_amd_mult_wave_number(int kind, int fp, void* wA, const void* A, const
   scale = 1.0 / scale;
   return fp == ACCFFT_AMD_F32 ? afft_mult_f32(kind, wA, A, n, osize, ostart, scale, stream)
                               : afft_mult_f64(kind, wA, A, n, osize, ostart, scale, stream);
+}
+
+int accfft_amd_poisson_filter(int fp, void* data_hat, const int n[3], const int osize[3],
+                              const int ostart[3], int method, void* stream) {
+  if (method != 0 && method != 1) return ACCFFT_AMD_ERR_INVALID;
+  return accfft_amd_mult_wave_number(method == 0 ? afft::SYM_POISSON_DISCRETE
+                                                 : afft::SYM_POISSON_CONTINUOUS,
+                                     fp, data_hat, data_hat, n, osize, ostart, 7u, stream);
 }
 
 int accfft_amd_axpy(int fp, int64_t n, double alpha, const void* x, void* y, void* stream) {

@@ -432,7 +432,12 @@ enum SymKind {
   SYM_LAPLACE = 3,
   SYM_BIHARMONIC = 4,
   SYM_INV_LAPLACE = 5,
-  SYM_INV_BIHARMONIC = 6
+  SYM_INV_BIHARMONIC = 6,
+  // Poisson Fourier filter of the step4 solver (steps/step4/kernels.cu:24-110),
+  // unit box, divided by the FFT size: method 0 = the discrete 7-point
+  // Laplacian's eigenvalues, method 1 = the continuous -4 pi^2 |k|^2
+  SYM_POISSON_DISCRETE = 7,
+  SYM_POISSON_CONTINUOUS = 8
 };
 
 // w = X; if X > N/2: w -= N; if X == N/2: w = 0  (operators_cuda.cu:32-37)
@@ -461,6 +466,27 @@ __device__ __forceinline__ cpx<T> apply_sym(int kind, long X, long Y, long Z, lo
                                       : wave_nyq0(Z, N2);
     const T f = (T)(scale * (double)w);
     return {-f * v.y, f * v.x};
+  }
+  if (kind == SYM_POISSON_DISCRETE || kind == SYM_POISSON_CONTINUOUS) {
+    // every mode but (0, 0, 0) divided by its eigenvalue times N0 N1 N2
+    // (1 / scale); the mean is set to zero.  Method 1 shifts k > N/2 (a
+    // floating comparison: the even-N Nyquist mode stays +N/2).
+    if (X == 0 && Y == 0 && Z == 0) return {(T)0, (T)0};
+    double ev;
+    if (kind == SYM_POISSON_DISCRETE) {
+      const double tp = 6.283185307179586476925;
+      const double n0 = (double)N0, n1 = (double)N1, n2 = (double)N2;
+      ev = 2.0 * ((cos(tp * (double)X / n0) - 1.0) * (n0 * n0) +
+                  (cos(tp * (double)Y / n1) - 1.0) * (n1 * n1) +
+                  (cos(tp * (double)Z / n2) - 1.0) * (n2 * n2));
+    } else {
+      const double kx = (double)X > 0.5 * (double)N0 ? (double)(X - N0) : (double)X;
+      const double ky = (double)Y > 0.5 * (double)N1 ? (double)(Y - N1) : (double)Y;
+      const double kz = (double)Z > 0.5 * (double)N2 ? (double)(Z - N2) : (double)Z;
+      ev = -39.47841760435743447534 * (kx * kx + ky * ky + kz * kz);  // -4 pi^2 |k|^2
+    }
+    const double den = ev / scale;
+    return {(T)((double)v.x / den), (T)((double)v.y / den)};
   }
   double f;
   if (kind == SYM_INV_BIHARMONIC) {

@@ -167,6 +167,14 @@ int accfft_amd_biharmonic(accfft_amd_plan* plan, void* BA, const void* A, double
 int accfft_amd_inv_laplace(accfft_amd_plan* plan, void* out, const void* A, double* timer);
 /* Inverse biharmonic: CPU-only upstream (src/operators.txx:336-392). */
 int accfft_amd_inv_biharmonic(accfft_amd_plan* plan, void* out, const void* A, double* timer);
+/* FFT Poisson solve of the step4 program (steps/step4/step4_gpu.cpp:520-640:
+ * accfft_execute_r2c_gpu, poisson_fourier_filter_gpu, accfft_execute_c2r_gpu)
+ * on an r2c plan, with the filter fused into the x pass: phi = the
+ * zero-mean solution of laplace(phi) = rho on the unit box.  method 0: the
+ * discrete 7-point Laplacian's eigenvalues; 1: -4 pi^2 |k|^2
+ * (steps/step4/kernels.cu:24-110).  rho is not modified. */
+int accfft_amd_poisson_solve(accfft_amd_plan* plan, int method, void* phi, const void* rho,
+                             double* timer);
 /* accfft_divergence_gpu(f) (operators_gpu.txx:448-553); slow != 0 gives
  * accfft_divergence_gpu_slow_t (:336-446). */
 int accfft_amd_divergence(accfft_amd_plan* plan, void* divA, const void* Ax,
@@ -176,13 +184,21 @@ int accfft_amd_divergence(accfft_amd_plan* plan, void* divA, const void* Ax,
  * src/operators_cuda.cu:421-486 (grad_mult_wave_number{x,y,z}_gpu_c(f),
  * laplace_mult_wave_number_gpu_c(f), biharmonic_mult_wave_number_gpu_c(f))
  * plus the inverse symbols.  kind: 0/1/2 grad x/y/z, 3 laplace,
- * 4 biharmonic, 5 inverse laplace, 6 inverse biharmonic.  scale_xyz selects
+ * 4 biharmonic, 5 inverse laplace, 6 inverse biharmonic, 7 / 8 the Poisson
+ * filter of methods 0 / 1 (accfft_amd_poisson_filter).  scale_xyz selects
  * the axes whose lengths divide the result (the grad launchers' bitset,
  * operators_cuda.cu:219-229; the other kinds use all three). */
 int accfft_amd_mult_wave_number(int kind, int fp, void* wA, const void* A,
                                 const int n[3], const int osize[3],
                                 const int ostart[3], unsigned scale_xyz,
                                 void* stream);
+/* poisson_fourier_filter_gpu (steps/step4/kernels.cu:24-146): in place on a
+ * half spectrum block osize at offset ostart of an n[0] x n[1] x n[2] r2c
+ * transform, every mode but (0,0,0) divided by its Poisson eigenvalue times
+ * n[0] n[1] n[2], the mean set to zero.  method 0 or 1 as
+ * accfft_amd_poisson_solve.  Stream-ordered. */
+int accfft_amd_poisson_filter(int fp, void* data_hat, const int n[3], const int osize[3],
+                              const int ostart[3], int method, void* stream);
 /* daxpy_gpu_c(f) (operators_cuda.cu:470-474): y += alpha * x. */
 int accfft_amd_axpy(int fp, int64_t n, double alpha, const void* x, void* y,
                     void* stream);
@@ -229,6 +245,28 @@ int accfft_amd_local_transpose_col(int r, int c, int64_t t, int64_t t2,
  * cycle-following. */
 int accfft_amd_local_transpose_inplace(int r, int c, int64_t t, void* A,
                                        int elem_size, void* stream);
+/* memcpy_v1_h1 (src/transpose_cuda.cu:97-143, called by the pencil
+ * transposes at src/transpose_gpu.cpp:1888, :2284): pack for an exchange
+ * over nprocs_1 ranks.  For each rank p, batch h < howmany and row
+ * i < local_n0, the local_n1_proc[p] tuples of n_tuples elements that start
+ * at column local_1_start_proc[p] of row i (data[h * idist + (i * N1 +
+ * start) * n_tuples]) go to the next place of send_recv, in the order
+ * (p, h, i).  The per-rank arrays are host memory; nprocs_1 <= 64.
+ * Bit-exact copy of elem_size-byte elements, stream-ordered. */
+int accfft_amd_pencil_pack(int nprocs_1, int howmany, int local_n0, int64_t n_tuples,
+                           const int64_t* local_n1_proc, void* send_recv, const void* data,
+                           int64_t idist, int64_t N1, const int64_t* local_1_start_proc,
+                           int elem_size, void* stream);
+/* memcpy_v1_h2 (src/transpose_cuda.cu:145-197, called at
+ * src/transpose_gpu.cpp:2118, :2414): unpack after the exchange over
+ * nprocs_0 ranks.  send_recv holds segments of local_n1 * n_tuples elements
+ * in the order (p, h, i) for i in [local_0_start_proc[p],
+ * local_0_start_proc[p] + local_n0_proc[p]); each goes to data[h * odist +
+ * i * local_n1 * n_tuples].  Host per-rank arrays; nprocs_0 <= 64. */
+int accfft_amd_pencil_unpack(int nprocs_0, int howmany, const int64_t* local_0_start_proc,
+                             const int64_t* local_n0_proc, void* data, int64_t odist,
+                             int64_t local_n1, int64_t n_tuples, const void* send_recv,
+                             int elem_size, void* stream);
 
 /* ---------------------------------------------------------------- misc */
 /* Synchronise the plan's stream (what the reference's blocking calls do at

@@ -24,7 +24,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
 _lib = None
 
-GRAD_X, GRAD_Y, GRAD_Z, LAPLACE, BIHARMONIC, INV_LAPLACE, INV_BIHARMONIC = range(7)
+(GRAD_X, GRAD_Y, GRAD_Z, LAPLACE, BIHARMONIC, INV_LAPLACE, INV_BIHARMONIC, POISSON_DISCRETE,
+ POISSON_CONTINUOUS) = range(9)
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
@@ -220,6 +221,19 @@ def inv_biharmonic(a):
     return spectral_op(INV_BIHARMONIC, a)
 
 
+def poisson_solve(rho, method: int = 0):
+    """The step4 FFT Poisson solve, steps/step4/step4_gpu.cpp:520-640: r2c,
+    poisson_fourier_filter (steps/step4/kernels.cu:24-110), c2r."""
+    return spectral_op(POISSON_DISCRETE + int(method), rho)
+
+
+def poisson_filter(h, N, ostart=(0, 0, 0), method: int = 0) -> np.ndarray:
+    """poisson_fourier_filter_gpu on a half spectrum block at ``ostart``
+    (steps/step4/kernels.cu:24-146); returns the filtered copy."""
+    N = _n3(N)
+    return mult(POISSON_DISCRETE + int(method), N, h, ostart, 1.0 / float(np.prod(N)))
+
+
 def divergence(ax, ay, az, slow: bool = False):
     """accfft_divergence_gpu_t (:448-553) or ``_slow_t`` (:336-446)."""
     ax, ay, az = _r(ax), _r(ay), _r(az)
@@ -321,6 +335,40 @@ def local_transpose_col(r, c, t, t2, a) -> np.ndarray:
 
 
 # ---------------------------------------------------------------- sizes
+def pencil_pack(nprocs_1, howmany, local_n0, n_tuples, local_n1_proc, data, idist, N1,
+                local_1_start_proc) -> np.ndarray:
+    """memcpy_v1_h1, src/transpose_cuda.cu:97-143: segments (proc, h, i),
+    each the local_n1_proc[proc] tuples of row i of batch h starting at
+    column local_1_start_proc[proc], concatenated."""
+    data = np.asarray(data).ravel()
+    out = []
+    for p in range(nprocs_1):
+        for h in range(howmany):
+            for i in range(local_n0):
+                src = h * idist + (i * N1 + local_1_start_proc[p]) * n_tuples
+                out.append(data[src:src + n_tuples * local_n1_proc[p]])
+    return np.concatenate(out) if out else np.zeros(0, data.dtype)
+
+
+def pencil_unpack(nprocs_0, howmany, local_0_start_proc, local_n0_proc, data, odist, local_n1,
+                  n_tuples, send_recv) -> np.ndarray:
+    """memcpy_v1_h2, src/transpose_cuda.cu:145-197: consecutive segments of
+    local_n1 * n_tuples elements of ``send_recv``, (proc, h, i) for i in
+    proc's row range, written to data[h * odist + i * local_n1 * n_tuples];
+    returns the updated copy of ``data``."""
+    data = np.array(data, copy=True).ravel()
+    send_recv = np.asarray(send_recv).ravel()
+    seg = local_n1 * n_tuples
+    ptr = 0
+    for p in range(nprocs_0):
+        for h in range(howmany):
+            for i in range(local_0_start_proc[p], local_0_start_proc[p] + local_n0_proc[p]):
+                dst = h * odist + i * seg
+                data[dst:dst + seg] = send_recv[ptr:ptr + seg]
+                ptr += seg
+    return data
+
+
 def local_size_r2c(n, elem: int, p=(1, 1), coords=(0, 0)):
     """accfft_local_size_dft_r2c_t<T>, src/accfft_common.cpp:98-192.
     Returns (alloc_max_bytes, isize, istart, osize, ostart)."""

@@ -34,6 +34,9 @@
  * fixtures, brute-force O(n^2) DFTs, closed forms and invariants.
  */
 #include <math.h>
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -402,7 +405,9 @@ enum {
   ORC_LAPLACE = 3,
   ORC_BIHARMONIC = 4,
   ORC_INV_LAPLACE = 5,
-  ORC_INV_BIHARMONIC = 6
+  ORC_INV_BIHARMONIC = 6,
+  ORC_POISSON_DISCRETE = 7,
+  ORC_POISSON_CONTINUOUS = 8
 };
 
 /* out = symbol(kind) * scale * in over a complex block of osize with global
@@ -426,6 +431,35 @@ ORC_API void orc_mult(int kind, const int *N, const int *osize,
                                     : wave_nyq0(Z, N[2]);
       out[2 * ptr] = -scale * w * im;
       out[2 * ptr + 1] = scale * w * re;
+      continue;
+    }
+    if (kind == ORC_POISSON_DISCRETE || kind == ORC_POISSON_CONTINUOUS) {
+      /* Poisson Fourier filter, steps/step4/kernels.cu:24-110 (unit box,
+       * dx = 1/NX): method 0 = 2 sum (cos(2 pi k/N) - 1)/dx^2, method 1 =
+       * -4 pi^2 sum kk^2 with kk = k - N when k > N/2 (a double comparison);
+       * times NX NY NZ (here 1/scale); the (0,0,0) mode is set to zero. */
+      if (X == 0 && Y == 0 && Z == 0) {
+        out[2 * ptr] = 0.0;
+        out[2 * ptr + 1] = 0.0;
+        continue;
+      }
+      double NX = N[0], NY = N[1], NZ = N[2];
+      double dx = 1.0 / NX, dy = 1.0 / NY, dz = 1.0 / NZ;
+      double sf;
+      if (kind == ORC_POISSON_DISCRETE) {
+        sf = 2 * ((cos(2 * M_PI * X / NX) - 1) / (dx * dx) +
+                  (cos(2 * M_PI * Y / NY) - 1) / (dy * dy) +
+                  (cos(2 * M_PI * Z / NZ) - 1) / (dz * dz));
+      } else {
+        double kx = X, ky = Y, kz = Z;
+        if (kx > NX / 2) kx -= NX;
+        if (ky > NY / 2) ky -= NY;
+        if (kz > NZ / 2) kz -= NZ;
+        sf = -4 * M_PI * M_PI * (kx * kx + ky * ky + kz * kz);
+      }
+      sf /= scale;
+      out[2 * ptr] = re / sf;
+      out[2 * ptr + 1] = im / sf;
       continue;
     }
     double f;
@@ -498,7 +532,8 @@ ORC_API void orc_grad_slow(const int *n, const double *A, double *Ax,
 
 /* laplace (:280-334), biharmonic (:555-609), inverse laplace
  * (src/operators.txx:1270-1321), inverse biharmonic (operators.txx:336-392
- * multiplier): 3-D r2c, multiply with scale 1/(N0 N1 N2), 3-D c2r. */
+ * multiplier), the step4 Poisson solve (steps/step4/step4_gpu.cpp:520-640):
+ * 3-D r2c, multiply with scale 1/(N0 N1 N2), 3-D c2r. */
 ORC_API void orc_spectral_op(int kind, const int *n, const double *A,
                              double *out) {
   const long N0 = n[0], N1 = n[1], N2 = n[2], Nh = N2 / 2 + 1;

@@ -7,6 +7,10 @@
 //     r2c then c2r, relative L1 of c2r(r2c(a))/N^3 - a below 1e-10 (FP64)
 //     or 1e-5 (FP32);
 //   step3 (steps/step3/step3_gpu.cpp): the same for c2c;
+//   step4 (steps/step4/step4_gpu.cpp:520-640, kernels.cu:24-146): Poisson
+//     solve of the sine case r2c -> poisson_fourier_filter_gpu -> c2r against
+//     its exact solution (method 1) and the discrete one (method 0), and the
+//     fused solve;
 //   step5 (steps/step5/step5_gpu.cpp:20-133, :559-699): grad, laplace,
 //     biharmonic and divergence of sin x sin y sin z + exp(-32|x-pi|^2)
 //     against closed forms (relative L1 < 1e-10 in FP64), plus the GPU
@@ -23,6 +27,7 @@
 #include "accfft_gpu.h"
 #include "accfft_gpuf.h"
 #include "accfft_operators_gpu.h"
+#include "accfft_poisson_gpu.h"
 #include "transpose_cuda.h"
 
 static int g_fail = 0;
@@ -207,6 +212,48 @@ static void step3_fp64(int* n) {
   (void)hipFree(data_hat);
 }
 
+// ------------------------------------------------------------------ step4
+static void step4_fp64(int* n) {
+  const size_t N = (size_t)n[0] * n[1] * n[2];
+  std::vector<double> rho(N);
+  for (int i = 0; i < n[0]; ++i)
+    for (int j = 0; j < n[1]; ++j)
+      for (int k = 0; k < n[2]; ++k)
+        rho[((size_t)i * n[1] + j) * n[2] + k] = sin(2 * M_PI * i / n[0]) *
+                                                 sin(2 * M_PI * j / n[1]) *
+                                                 sin(2 * M_PI * k / n[2]);
+  int isize[3], istart[3], osize[3], ostart[3];
+  MPI_Comm c_comm;
+  int c_dims[2] = {0, 0};
+  accfft_create_comm(MPI_COMM_WORLD, c_dims, &c_comm);
+  const int alloc_max = accfft_local_size_dft_r2c_gpu(n, isize, istart, osize, ostart, c_comm);
+  double* d_rho = to_dev<double>(rho);
+  double* d_phi = nullptr;
+  Complex* phi_hat = nullptr;
+  (void)hipMalloc((void**)&d_phi, N * sizeof(double));
+  (void)hipMalloc((void**)&phi_hat, alloc_max);
+  accfft_plan_gpu* plan = accfft_plan_dft_3d_r2c_gpu(n, d_rho, (double*)phi_hat, c_comm, ACCFFT_MEASURE);
+  double ev0 = 0;
+  for (int d = 0; d < 3; ++d) ev0 += 2 * (cos(2 * M_PI / n[d]) - 1) * n[d] * n[d];
+  const double ev[2] = {ev0, -12 * M_PI * M_PI};
+  const char* names[2] = {"step4 Poisson method 0 FP64", "step4 Poisson method 1 FP64"};
+  for (int m = 0; m < 2; ++m) {
+    std::vector<double> exact(N);
+    for (size_t q = 0; q < N; ++q) exact[q] = rho[q] / ev[m];
+    accfft_execute_r2c_gpu(plan, d_rho, phi_hat);
+    poisson_fourier_filter_gpu(phi_hat, n, osize, ostart, m);
+    accfft_execute_c2r_gpu(plan, phi_hat, d_phi);
+    report(names[m], rel_l1(to_host(d_phi, N), exact), 1e-10);
+    accfft_poisson_solve_gpu(d_phi, d_rho, plan, m);
+    report(m ? "step4 fused Poisson solve method 1" : "step4 fused Poisson solve method 0",
+           rel_l1(to_host(d_phi, N), exact), 1e-10);
+  }
+  accfft_destroy_plan_gpu(plan);
+  (void)hipFree(d_rho);
+  (void)hipFree(d_phi);
+  (void)hipFree(phi_hat);
+}
+
 // ------------------------------------------------------------------ step5
 static void step5_fp64(int* n) {
   MPI_Comm c_comm;
@@ -338,6 +385,8 @@ int main(int argc, char** argv) {
   int n2[3] = {32, 48, 30};
   step2_fp64(n2);
   step3_fp64(n2);
+  int n4[3] = {64, 48, 40};
+  step4_fp64(n4);
   step5_fp64(n5);
   step5_fp32(n5);
   transposes();

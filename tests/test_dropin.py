@@ -1,7 +1,7 @@
 """The drop-in C++ API (include/accfft_gpu.h, accfft_gpuf.h,
 accfft_operators_gpu.h, transpose_cuda.h): a program written against the
 reference's function names (tests/cpp/dropin_steps.cpp, modelled on
-steps/step1, step2, step3 and step5) must compile unchanged and pass the
+steps/step1 to step5) must compile unchanged and pass the
 reference programs' own correctness thresholds on the GPU."""
 import os
 import subprocess
@@ -17,7 +17,7 @@ def test_dropin_headers_compile(tmp_path):
     src = tmp_path / "t.cpp"
     src.write_text(
         '#include "accfft_gpu.h"\n#include "accfft_gpuf.h"\n#include "accfft_operators_gpu.h"\n'
-        '#include "transpose_cuda.h"\n'
+        '#include "accfft_poisson_gpu.h"\n#include "transpose_cuda.h"\n'
         "int main(){int n[3]={8,8,8},i[3],s[3],o[3],t[3];MPI_Comm c;int d[2]={0,0};"
         "accfft_create_comm(MPI_COMM_WORLD,d,&c);"
         "return accfft_local_size_dft_r2c_gpu(n,i,s,o,t,c) == 8*8*10*8 ? 0 : 1;}\n")
