@@ -346,26 +346,50 @@ struct Ops {
   // is the y pass's store, the exchange is the plan's hook (identity when
   // none: the packed send buffer is the receive buffer), the x pass reads
   // the received [N0][N1/P][Nh] in place.
+  // Calls the plan's exchange hook; in a timed call its span is recorded
+  // for timer[2] (the reference's communication time, doc/timing.md).
+  static int exchange(accfft_amd_plan* p, const void* send, void* recv, int64_t block, int dir) {
+    if (p->timing) {
+      if ((!p->evx0 && hipEventCreate(&p->evx0) != hipSuccess) ||
+          (!p->evx1 && hipEventCreate(&p->evx1) != hipSuccess))
+        return ACCFFT_AMD_ERR_HIP;
+      (void)hipEventRecord(p->evx0, p->stream);
+    }
+    const int st = p->exchange(send, recv, block, p->nprocs, dir, p->stream, p->exchange_user);
+    if (p->timing) {
+      (void)hipEventRecord(p->evx1, p->stream);
+      p->comm_recorded = true;
+    }
+    return st;
+  }
+
+  // Row pitch of the plan-internal z-y workspace: Nh complex rounded up to
+  // 128 bytes when that costs at most 2 % more memory, so that the y pass's
+  // row segments do not straddle cache lines (the user-visible layouts keep
+  // the reference's pitch Nh).
+  static int64_t ws_pitch(int64_t Nh) {
+    const int64_t q = 128 / (int64_t)sizeof(C);
+    const int64_t np = (Nh + q - 1) / q * q;
+    return (np - Nh) * 50 <= Nh ? np : Nh;
+  }
   static int r2c_slab(accfft_amd_plan* p, const void* in, void* out) {
     const int64_t N0 = p->n[0], N1 = p->n[1], N2 = p->n[2], Nh = N2 / 2 + 1;
     const int P = p->nprocs;
-    const int64_t n0 = N0 / P, n1 = N1 / P;
+    const int64_t n0 = N0 / P, n1 = N1 / P, Np = ws_pitch(Nh);
     const size_t cb = (size_t)n0 * N1 * Nh * sizeof(C);
     int st;
-    if ((st = ensure_ws(p, 0, cb))) return st;
+    if ((st = ensure_ws(p, 0, (size_t)n0 * N1 * Np * sizeof(C)))) return st;
     if (p->exchange && (st = ensure_ws(p, 1, cb))) return st;
     void* zbuf = p->ws[0];
     void* send = p->exchange ? p->ws[1] : out;
-    auto az = zargs(in, zbuf, n0 * N1, N2, Nh);
+    auto az = zargs(in, zbuf, n0 * N1, N2, Np);
     if ((st = run_pass<T>(p, "z", false, M_R2C, -1, az, tab(p, 2), 0))) return st;
-    auto ay = sargs(zbuf, send, n0 * Nh, Nh, N1 * Nh, n1 * Nh, Nh, Nh);
+    auto ay = sargs(zbuf, send, n0 * Nh, Nh, N1 * Np, n1 * Nh, Np, Nh);
     ay.oshift = ilog2(n1);
     ay.Bout = n0 * n1 * Nh;
     if (P == 1) ay.oshift = 0;
     if ((st = run_pass<T>(p, "y", true, M_C2C, -1, ay, tab(p, 1), 1))) return st;
-    if (p->exchange &&
-        (st = p->exchange(send, out, (int64_t)(n0 * n1 * Nh * sizeof(C)), P, -1, p->stream,
-                          p->exchange_user)))
+    if (p->exchange && (st = exchange(p, send, out, (int64_t)(n0 * n1 * Nh * sizeof(C)), -1)))
       return st;
     auto ax = sargs(out, out, n1 * Nh, n1 * Nh, 0, 0, n1 * Nh, n1 * Nh);
     return run_pass<T>(p, "x", true, M_C2C, -1, ax, tab(p, 0), 2);
@@ -376,27 +400,25 @@ struct Ops {
   static int c2r_slab(accfft_amd_plan* p, void* in, void* out) {
     const int64_t N0 = p->n[0], N1 = p->n[1], N2 = p->n[2], Nh = N2 / 2 + 1;
     const int P = p->nprocs;
-    const int64_t n0 = N0 / P, n1 = N1 / P;
+    const int64_t n0 = N0 / P, n1 = N1 / P, Np = ws_pitch(Nh);
     const size_t cb = (size_t)n0 * N1 * Nh * sizeof(C);
     int st;
-    if ((st = ensure_ws(p, 0, cb))) return st;
+    if ((st = ensure_ws(p, 0, (size_t)n0 * N1 * Np * sizeof(C)))) return st;
     if (p->exchange && (st = ensure_ws(p, 1, cb))) return st;
     auto ax = sargs(in, in, n1 * Nh, n1 * Nh, 0, 0, n1 * Nh, n1 * Nh);
     if ((st = run_pass<T>(p, "x", true, M_C2C, +1, ax, tab(p, 0), 3))) return st;
     const void* recv = in;
     if (p->exchange) {
-      if ((st = p->exchange(in, p->ws[1], (int64_t)(n0 * n1 * Nh * sizeof(C)), P, +1, p->stream,
-                            p->exchange_user)))
-        return st;
+      if ((st = exchange(p, in, p->ws[1], (int64_t)(n0 * n1 * Nh * sizeof(C)), +1))) return st;
       recv = p->ws[1];
     }
     void* ybuf = p->ws[0];
-    auto ay = sargs(recv, ybuf, n0 * Nh, Nh, n1 * Nh, N1 * Nh, Nh, Nh);
+    auto ay = sargs(recv, ybuf, n0 * Nh, Nh, n1 * Nh, N1 * Np, Nh, Np);
     ay.ishift = ilog2(n1);
     ay.Bin = n0 * n1 * Nh;
     if (P == 1) ay.ishift = 0;
     if ((st = run_pass<T>(p, "y", true, M_C2C, +1, ay, tab(p, 1), 4))) return st;
-    auto az = zargs(ybuf, out, n0 * N1, Nh, N2);
+    auto az = zargs(ybuf, out, n0 * N1, Np, N2);
     return run_pass<T>(p, "z", false, M_C2R, +1, az, tab(p, 2), 5);
   }
   static int ilog2(int64_t v) {

@@ -38,6 +38,11 @@ struct accfft_amd_plan {
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_ms = 0.f;
+  // timer contract (timer[2] = exchange, timer[4] = local FFT): set while a
+  // timed call runs; the slab plan brackets its exchange hook with evx0/1
+  bool timing = false;
+  bool comm_recorded = false;
+  hipEvent_t evx0 = nullptr, evx1 = nullptr;
   int device = 0;
   int split = 0;  // afft::SplitKind of the radix-16 stages (FP32 plans)
   // per-launch event profiling (accfft_amd_plan_set_profiling)

@@ -132,7 +132,9 @@ int accfft_amd_plan_data(const accfft_amd_plan* plan, void** data, void** data_o
 /* accfft_execute_r2c_gpu(f) (src/accfft_gpu.cpp:635-653) and
  * accfft_execute_gpu(dir=-1) (:444-539).  xyz bit d selects axis d (bit 0 =
  * x), as std::bitset<3>.  NULL pointers take the plan's pointers.  timer[4]
- * accumulates the FFT time in seconds (doc/timing.md). */
+ * accumulates the local FFT time in seconds and timer[2] the time of a slab
+ * plan's exchange hook (communication), as doc/timing.md; every other call
+ * that takes a timer adds its time to timer[4]. */
 int accfft_amd_execute_r2c(accfft_amd_plan* plan, const void* in, void* out,
                            double* timer, unsigned xyz);
 /* accfft_execute_c2r_gpu(f) (:655-675), accfft_execute_gpu(dir=+1)

@@ -78,16 +78,21 @@ def test_slab_exchange_hook(dev):
         return 0 if hip.hipMemcpyAsync(recv, send, nbytes * nprocs, 3, stream) == 0 else 5
 
     plan.set_exchange(hook)
-    plan.execute_r2c(ad, h1)
+    timer = [0.0] * 5
+    plan.execute_r2c(ad, h1, timer=timer)
     torch.cuda.synchronize()
     assert torch.equal(h0, h1)
+    # doc/timing.md: the exchange in timer[2], the local FFT in timer[4]
+    assert timer[2] > 0 and timer[4] > 0 and timer[0] == timer[1] == timer[3] == 0, timer
     b0, b1 = torch.empty_like(ad), torch.empty_like(ad)
     hc = h1.clone()
     plan.execute_c2r(h1, b1)
     plan.set_exchange(None)
-    plan.execute_c2r(hc, b0)
+    timer = [0.0] * 5
+    plan.execute_c2r(hc, b0, timer=timer)
     torch.cuda.synchronize()
     assert torch.equal(b0, b1)
+    assert timer[2] == 0 and timer[4] > 0, timer  # identity exchange: no communication
     blk = n0 * n1 * Nh * 16
     assert calls == [(blk, P, -1), (blk, P, 1)]
     plan.destroy()
