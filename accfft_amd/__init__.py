@@ -107,6 +107,8 @@ def lib() -> ctypes.CDLL:
     L.accfft_amd_adjoint_grad.argtypes = [_vp, _vp, _vp, _vp, _vp, dp]
     L.accfft_amd_adjoint_divergence.argtypes = [_vp, _vp, _vp, _vp, _vp, dp]
     L.accfft_amd_plan_release_workspace.argtypes = [_vp]
+    L.accfft_amd_plan_get_tuning.argtypes = [_vp, _i3]
+    L.accfft_amd_plan_set_tuning.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
     L.accfft_amd_plan_workspace_bytes.argtypes = [_vp, i64p]
     L.accfft_amd_last_elapsed_ms.argtypes = [_vp, ctypes.POINTER(ctypes.c_float)]
     L.accfft_amd_plan_set_profiling.argtypes = [_vp, ctypes.c_int]
@@ -350,6 +352,15 @@ class Plan:
         _check(lib().accfft_amd_adjoint_divergence(self._bind(), _ptr(ox), _ptr(oy), _ptr(oz),
                                                    _ptr(v), tp), "adjoint_divergence")
         self._untimer(timer, arr)
+
+    def tuning(self):
+        """Tile configuration per axis (-1 rule, -2 one-level, v >= 0 variant)."""
+        c = _i3()
+        _check(lib().accfft_amd_plan_get_tuning(self._bind(), c), "plan_get_tuning")
+        return tuple(c)
+
+    def set_tuning(self, axis: int, choice: int):
+        _check(lib().accfft_amd_plan_set_tuning(self._bind(), axis, choice), "plan_set_tuning")
 
     def release_workspace(self):
         _check(lib().accfft_amd_plan_release_workspace(self._bind()), "release_workspace")

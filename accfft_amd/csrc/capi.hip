@@ -117,6 +117,14 @@ void free_plan(accfft_amd_plan* p) {
   delete p;
 }
 
+// ACCFFT_MEASURE / ACCFFT_PATIENT: time the tile configurations of the
+// strided passes on this plan's shapes (doc/flags.md); ACCFFT_ESTIMATE
+// keeps the built-in rule.
+int maybe_tune(accfft_amd_plan* p) {
+  if (!(p->flags & (ACCFFT_AMD_MEASURE | ACCFFT_AMD_PATIENT))) return ACCFFT_AMD_OK;
+  return p->fp == ACCFFT_AMD_F32 ? afft_tune_f32(p) : afft_tune_f64(p);
+}
+
 int make_plan(int kind, const int n[3], void* data, void* data_out, int fp, unsigned flags,
               accfft_amd_plan** out) {
   if (!out) return ACCFFT_AMD_ERR_NULL;
@@ -154,7 +162,8 @@ int make_plan(int kind, const int n[3], void* data, void* data_out, int fp, unsi
     free_plan(p);
     return ACCFFT_AMD_ERR_HIP;
   }
-  const int st = fp == ACCFFT_AMD_F32 ? afft_tables_f32(p) : afft_tables_f64(p);
+  int st = fp == ACCFFT_AMD_F32 ? afft_tables_f32(p) : afft_tables_f64(p);
+  if (!st && kind != 3) st = maybe_tune(p);
   if (st) {
     free_plan(p);
     return st;
@@ -259,7 +268,34 @@ int accfft_amd_plan_dft_3d_r2c_slab(const int n[3], int nprocs, void* data, void
   p->nprocs = nprocs;
   p->alloc_max = size_r2c(n, p->isize, p->istart, p->osize, p->ostart, nprocs, 1, 0, 0,
                           elem_of(fp));
+  if ((st = maybe_tune(p))) {
+    free_plan(p);
+    return st;
+  }
   *plan = p;
+  return ACCFFT_AMD_OK;
+}
+
+int accfft_amd_plan_get_tuning(const accfft_amd_plan* p, int choice[3]) {
+  if (!p || !choice) return ACCFFT_AMD_ERR_NULL;
+  for (int ax = 0; ax < 3; ++ax)
+    choice[ax] = p->fp == ACCFFT_AMD_F32 ? p->tf[ax].s4v : p->td[ax].s4v;
+  return ACCFFT_AMD_OK;
+}
+
+int accfft_amd_plan_set_tuning(accfft_amd_plan* p, int axis, int choice) {
+  if (!p) return ACCFFT_AMD_ERR_NULL;
+  if (axis < 0 || axis > 2 || choice < -2) return ACCFFT_AMD_ERR_INVALID;
+  const int n = p->n[axis];
+  const bool has = p->fp == ACCFFT_AMD_F32 ? p->tf[axis].tw_sub != nullptr
+                                           : p->td[axis].tw_sub != nullptr;
+  if (choice >= 0) {
+    const bool ok = p->fp == ACCFFT_AMD_F32 ? afft::s4_config_ok<float>(n, choice)
+                                            : afft::s4_config_ok<double>(n, choice);
+    if (!ok || !has) return ACCFFT_AMD_ERR_INVALID;
+  }
+  if (p->fp == ACCFFT_AMD_F32) p->tf[axis].s4v = choice;
+  else p->td[axis].s4v = choice;
   return ACCFFT_AMD_OK;
 }
 

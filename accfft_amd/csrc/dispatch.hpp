@@ -7,6 +7,7 @@ struct accfft_amd_plan;
 
 #define AFFT_DECLARE(SUF)                                                                   \
   int afft_tables_##SUF(accfft_amd_plan* p);                                                \
+  int afft_tune_##SUF(accfft_amd_plan* p);                                                  \
   int afft_r2c_##SUF(accfft_amd_plan* p, const void* in, void* out, unsigned xyz);          \
   int afft_c2r_##SUF(accfft_amd_plan* p, void* in, void* out, unsigned xyz);                \
   int afft_c2c_##SUF(accfft_amd_plan* p, int dir, void* in, void* out, unsigned xyz);       \

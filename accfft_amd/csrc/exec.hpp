@@ -13,6 +13,8 @@
 #include <stdio.h>
 #include <stdlib.h>
 
+#include <vector>
+
 #include "plan.hpp"
 
 namespace afft {
@@ -539,6 +541,110 @@ struct Ops {
     if ((st = status_of(launch_axpy<T>(R, (T)1, (const T*)p->ws[1], (T*)div, p->stream))))
       return st;
     return deriv(p, 2, Az, div, 1);
+  }
+
+  // ---------------------------------------------------------- MEASURE tuning
+  // The reference's ACCFFT_MEASURE tunes its local FFT and transposes at
+  // plan time (doc/flags.md).  Here, every strided C2C pass of the plan
+  // whose length has several tile configurations times each candidate
+  // (one-level tiles, the two-level variants) on its real shape over a
+  // zeroed temporary buffer, and keeps the fastest when it beats the
+  // built-in rule by more than 3 %.  If the buffer cannot be allocated the
+  // rule stays.  As with FFTW's MEASURE, the choice can change results in
+  // the last bits.
+  static int tune(accfft_amd_plan* p) {
+    if (p->split) return ACCFFT_AMD_OK;  // split plans keep the built-in tiles
+    const int64_t N0 = p->n[0], N1 = p->n[1], N2 = p->n[2], Nh = N2 / 2 + 1;
+    struct Pass {
+      int ax, order;
+      FftArgs<T> a;
+      int64_t in_elems, out_elems;
+    };
+    std::vector<Pass> passes;
+    if (p->kind == 1 || p->kind == 2) {
+      const int64_t L = p->kind == 1 ? Nh : N2, e = N0 * N1 * L;
+      passes.push_back({1, 1, y_args(p, nullptr, nullptr, L), e, 0});
+      passes.push_back({0, 2, x_args(p, nullptr, nullptr, L), e, 0});
+    } else if (p->kind == 3) {
+      const int P = p->nprocs;
+      const int64_t n0 = N0 / P, n1 = N1 / P, Np = ws_pitch(Nh);
+      auto ay = sargs(nullptr, nullptr, n0 * Nh, Nh, N1 * Np, n1 * Nh, Np, Nh);
+      ay.oshift = P == 1 ? 0 : ilog2(n1);
+      ay.Bout = n0 * n1 * Nh;
+      passes.push_back({1, 1, ay, n0 * N1 * Np, N0 * n1 * Nh});
+      passes.push_back({0, 2, sargs(nullptr, nullptr, n1 * Nh, n1 * Nh, 0, 0, n1 * Nh, n1 * Nh),
+                        N0 * n1 * Nh, 0});
+    }
+    int64_t elems = 0;
+    bool any = false;
+    for (const Pass& q : passes) {
+      const AxisTab<T>& t = tab(p, q.ax);
+      if (!t.is_static || !t.tw_sub || (t.n != 512 && t.n != 1024 && t.n != 2048)) continue;
+      any = true;
+      if (q.in_elems + q.out_elems > elems) elems = q.in_elems + q.out_elems;
+    }
+    if (!any) return ACCFFT_AMD_OK;
+    void* buf = nullptr;
+    if (hipMalloc(&buf, (size_t)elems * sizeof(C)) != hipSuccess) {
+      (void)hipGetLastError();
+      return ACCFFT_AMD_OK;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int st = ACCFFT_AMD_OK;
+    if (hipMemsetAsync(buf, 0, (size_t)elems * sizeof(C), p->stream) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+      st = ACCFFT_AMD_ERR_HIP;
+    for (const Pass& q : passes) {
+      if (st) break;
+      AxisTab<T>& t = const_cast<AxisTab<T>&>(tab(p, q.ax));
+      if (!t.is_static || !t.tw_sub || (t.n != 512 && t.n != 1024 && t.n != 2048)) continue;
+      FftArgs<T> a = q.a;
+      a.in = buf;
+      a.out = q.out_elems ? (void*)((C*)buf + q.in_elems) : buf;
+      // mean of 5 launches after one warm-up; < 0 when the candidate fails
+      auto measure = [&](int cand) -> float {
+        t.s4v = cand;
+        for (int r = 0; r < 6; ++r) {
+          if (r == 1) (void)hipEventRecord(e0, p->stream);
+          if (run_pass<T>(p, "tune", true, M_C2C, -1, a, t, q.order)) return -1.f;
+        }
+        (void)hipEventRecord(e1, p->stream);
+        float ms = -1.f;
+        if (hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+          return -1.f;
+        return ms / 5.f;
+      };
+      // candidates: the rule (-1), one-level tiles (-2), the variants; two
+      // interleaved rounds, each candidate's best kept (clock ramps and
+      // neighbours' noise hit every candidate alike)
+      float best_of[6];
+      for (float& b : best_of) b = -1.f;
+      for (int round = 0; round < 2; ++round)
+        for (int cand = -1; cand < 4; cand = cand == -1 ? -2 : cand == -2 ? 0 : cand + 1) {
+          if (cand >= 0 && !s4_config_ok<T>(t.n, cand)) continue;
+          const float ms = measure(cand);
+          float& b = best_of[cand + 2];
+          if (ms > 0.f && (b < 0.f || ms < b)) b = ms;
+        }
+      const float rule = best_of[1];
+      int best = -1;
+      float best_ms = rule;
+      for (int cand = -2; cand < 4; ++cand) {
+        const float ms = best_of[cand + 2];
+        if (cand != -1 && ms > 0.f && rule > 0.f && ms < best_ms) {
+          best_ms = ms;
+          best = cand;
+        }
+      }
+      t.s4v = (best != -1 && best_ms < 0.97f * rule) ? best : -1;
+    }
+    (void)hipGetLastError();
+    (void)hipStreamSynchronize(p->stream);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(buf);
+    return st;
   }
 };
 

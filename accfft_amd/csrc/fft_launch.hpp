@@ -19,6 +19,9 @@ struct AxisTab {
   const cpx<T>* tw_split = nullptr;   // per-stage table of the split schedule
   const cpx<T>* tw_sub = nullptr;     // per-stage table of length n/16 (two-level tiles)
   const cpx<T>* tw_subw = nullptr;    // the same for the SCHED_WIDE schedule
+  // strided C2C configuration picked by the plan's MEASURE tuning: -1 the
+  // built-in rule, -2 one-level tiles, v >= 0 two-level variant S4Cfg<T, n, v>
+  int s4v = -1;
 };
 
 // Lengths with a matrix-core (split-precision) schedule.
@@ -146,16 +149,28 @@ inline int ilog2i(int n) {
   return l;
 }
 template <typename T>
-inline bool four_step(int n) {
+inline bool four_step_default(int n) {
   static const unsigned mask =
       env_mask("ACCFFT_AMD_S4", sizeof(T) == 4 ? (1u << 9) | (1u << 10) | (1u << 11) : 0u);
   return (mask >> ilog2i(n)) & 1u;
 }
+inline bool four_step_env_set() {
+  static const bool set = getenv("ACCFFT_AMD_S4") != nullptr;
+  return set;
+}
+// Two-level tiles for this axis: ACCFFT_AMD_S4 if set, else the tuned
+// choice, else the built-in rule.
+template <typename T>
+inline bool four_step(const AxisTab<T>& tab) {
+  if (!four_step_env_set() && tab.s4v != -1) return tab.s4v >= 0;
+  return four_step_default<T>(tab.n);
+}
 // Variant of the two-level configuration for length n (S4Cfg<T, n, V>);
-// blk: rows in exchange blocks (the slab plan's y pass).  Default: the
-// persistent 2048 tiles for plain rows (measured 8 % faster on the 2048
-// x passes of the 256 x 2048 x 2048 slab), variant 0 otherwise.
-inline int four_step_variant(int n, bool blk) {
+// blk: rows in exchange blocks (the slab plan's y pass); tuned: the plan's
+// MEASURE choice (-1 none).  ACCFFT_AMD_S4V overrides; the built-in rule is
+// the persistent 2048 tiles for plain rows (measured 8 % faster on the 2048
+// x passes of the 256 x 2048 x 2048 slab) and variant 0 otherwise.
+inline int four_step_variant(int n, bool blk, int tuned = -1) {
   static const std::vector<std::pair<int, int>> tab = [] {
     std::vector<std::pair<int, int>> t;
     const char* e = getenv("ACCFFT_AMD_S4V");
@@ -173,7 +188,26 @@ inline int four_step_variant(int n, bool blk) {
   }();
   for (const auto& kv : tab)
     if (kv.first == n) return kv.second;
+  if (tuned >= 0) return tuned;
   return (n == 2048 && !blk) ? 2 : 0;
+}
+
+// Whether S4Cfg<T, n, v> exists (the candidates of the MEASURE tuning).
+template <typename T>
+inline bool s4_config_ok(int n, int v) {
+#define AFFT_S4OK(NN)                                                        \
+  if (n == NN) switch (v) {                                                  \
+      case 0: return S4Cfg<T, NN, 0>::OK;                                    \
+      case 1: return S4Cfg<T, NN, 1>::OK;                                    \
+      case 2: return S4Cfg<T, NN, 2>::OK;                                    \
+      case 3: return S4Cfg<T, NN, 3>::OK;                                    \
+      default: return false;                                                 \
+    }
+  AFFT_S4OK(512)
+  AFFT_S4OK(1024)
+  AFFT_S4OK(2048)
+#undef AFFT_S4OK
+  return false;
 }
 
 // Compute units of the current device (read once per process).
@@ -223,7 +257,7 @@ hipError_t launch_s4v(int sign, FftArgs<T>& a, const AxisTab<T>& tab, hipStream_
 
 template <typename T, int N>
 hipError_t launch_s4(int sign, FftArgs<T>& a, const AxisTab<T>& tab, hipStream_t st) {
-  switch (four_step_variant(N, a.ishift != 0 || a.oshift != 0)) {
+  switch (four_step_variant(N, a.ishift != 0 || a.oshift != 0, tab.s4v)) {
     case 1: return launch_s4v<T, N, 1>(sign, a, tab, st);
     case 2: return launch_s4v<T, N, 2>(sign, a, tab, st);
     case 3: return launch_s4v<T, N, 3>(sign, a, tab, st);
@@ -245,7 +279,7 @@ hipError_t launch_fft(bool strided, int mode, int sign, FftArgs<T> a, const Axis
   if (tab.is_static) {
     a.tw = tab.tw_static;
     int per;
-    if (strided && mode == M_C2C && tab.tw_sub && four_step<T>(n)) {
+    if (strided && mode == M_C2C && tab.tw_sub && four_step<T>(tab)) {
       switch (n) {
         case 512: return launch_s4<T, 512>(sign, a, tab, st);
         case 1024: return launch_s4<T, 1024>(sign, a, tab, st);

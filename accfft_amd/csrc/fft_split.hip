@@ -44,7 +44,7 @@ static hipError_t launch_split_s4(int sign, FftArgs<float>& a, const AxisTab<flo
                                   hipStream_t st) {
   a.sc = tab.sc;
   a.sc.n = tab.n;
-  switch (four_step_variant(N, a.ishift != 0 || a.oshift != 0)) {
+  switch (four_step_variant(N, a.ishift != 0 || a.oshift != 0, tab.s4v)) {
 #define AFFT_SV(VV) \
   case VV:          \
     if constexpr (split_s4_ok<N, VV>()) return launch_s4v<float, N, VV, SPLIT>(sign, a, tab, st); \
@@ -75,7 +75,7 @@ hipError_t launch_fft_split(bool strided, int mode, int sign, FftArgs<float> a,
                             const AxisTab<float>& tab, int split, hipStream_t st) {
   const int64_t units = pair_mode(mode) ? (a.nlines + 1) / 2 : a.nlines;
   if (units <= 0) return hipSuccess;
-  if (strided && mode == M_C2C && tab.tw_sub && four_step<float>(tab.n) &&
+  if (strided && mode == M_C2C && tab.tw_sub && four_step<float>(tab) &&
       (tab.n == 512 || tab.n == 1024 || tab.n == 2048)) {
 #define AFFT_S4S(NN)                                                                    \
   if (tab.n == NN) switch (split) {                                                     \

@@ -64,7 +64,8 @@ def run_gpu(n, fp, steps, warmup, profile=True):
     x = torch.randn(n, generator=g, device=dev, dtype=rdt)
     spec = torch.empty((n[0], n[1], n[2] // 2 + 1), device=dev, dtype=cdt)
     back = torch.empty(n, device=dev, dtype=rdt)
-    plan = af.Plan("r2c", n, x, spec, fp)
+    plan = af.Plan("r2c", n, x, spec, fp)  # ACCFFT_MEASURE: tiles tuned at plan time
+    tuning = list(plan.tuning())
 
     def step():
         plan.execute_r2c(x, spec)
@@ -113,7 +114,7 @@ def run_gpu(n, fp, steps, warmup, profile=True):
         prof = plan.profile()
         plan.set_profiling(False)
     plan.destroy()
-    return ms, err, prof, copy_gbps
+    return ms, err, prof, copy_gbps, tuning
 
 
 SPLIT_FLAGS = {"bf16x2": 0x1000, "bf16x3": 0x2000, "fp16x2": 0x4000}
@@ -340,7 +341,7 @@ def main():
     import accfft_amd as af
 
     n = (args.n, args.n, args.n)
-    ms, err, prof, copy_gbps = run_gpu(n, af.F32, args.steps, args.warmup)
+    ms, err, prof, copy_gbps, tuning = run_gpu(n, af.F32, args.steps, args.warmup)
     log(f"fp32 {n} round trip {ms:.4f} ms, rel err {err:.3e}; device copy {copy_gbps:.0f} GB/s")
     for k, (tot, cnt, b) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
         log(f"  {k:40s} avg {tot / max(cnt, 1) * 1e3:8.1f} us  {b / (tot / max(cnt, 1)) / 1e6:8.1f} GB/s"
@@ -355,7 +356,7 @@ def main():
 
     extra = {}
     if not args.no_fp64:
-        ms64, err64, _, _ = run_gpu(n, af.F64, max(5, args.steps // 2), 2, profile=False)
+        ms64, err64, _, _, _ = run_gpu(n, af.F64, max(5, args.steps // 2), 2, profile=False)
         alg64, _ = alg_bytes_roundtrip(n, 8)
         extra = {"fp64_ms_per_step": round(ms64, 4), "fp64_rel_err": err64,
                  "fp64_pass_model_frac": round(alg64 / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
@@ -391,6 +392,7 @@ def main():
             "pass_model_frac_of_hbm_peak": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "compulsory_frac_of_hbm_peak": round(comp / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "device_copy_GBps": round(copy_gbps, 1),
+            "tile_tuning_xyz": tuning,
             "kernels": {k: {"avg_us": round(v[0] / max(v[1], 1) * 1e3, 2),
                             "alg_GBps": round(v[2] / (v[0] / max(v[1], 1) * 1e-3) / 1e9, 1)}
                         for k, v in prof.items()},

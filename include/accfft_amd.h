@@ -46,7 +46,13 @@ enum {
 enum { ACCFFT_AMD_F64 = 0, ACCFFT_AMD_F32 = 1 };
 
 /* Plan flags.  ACCFFT_ESTIMATE(1)/MEASURE(2)/PATIENT(4) of
- * include/accfft_common.h:40-42 are accepted and do not change results. */
+ * include/accfft_common.h:40-42 (doc/flags.md): with MEASURE or PATIENT the
+ * plan times the tile configurations of its strided passes on its own
+ * shapes over a temporary buffer and keeps the fastest (which can change
+ * results in the last bits); ESTIMATE keeps the built-in choice. */
+#define ACCFFT_AMD_ESTIMATE 1u
+#define ACCFFT_AMD_MEASURE 2u
+#define ACCFFT_AMD_PATIENT 4u
 #define ACCFFT_AMD_BLOCKING 0x100u       /* synchronise the stream after each call */
 #define ACCFFT_AMD_PRESERVE_INPUT 0x200u /* c2r / inverse c2c keep their input */
 /* FP32 plans only (no counterpart in the reference): radix-16 stages of
@@ -125,6 +131,13 @@ int accfft_amd_plan_set_stream(accfft_amd_plan* plan, void* stream);
 int accfft_amd_plan_info(const accfft_amd_plan* plan, int n[3], int isize[3],
                          int istart[3], int osize[3], int ostart[3],
                          int64_t* alloc_max, int* fp, int* kind, int* inplace);
+/* The tile configuration of each axis's strided C2C passes (no counterpart
+ * in the reference; what its MEASURE planning would record): -1 the
+ * built-in rule, -2 one-level tiles, v >= 0 two-level variant v.  get:
+ * what ACCFFT_MEASURE chose; set: force one (ERR_INVALID when the axis
+ * length has no such variant), e.g. to test each configuration. */
+int accfft_amd_plan_get_tuning(const accfft_amd_plan* plan, int choice[3]);
+int accfft_amd_plan_set_tuning(accfft_amd_plan* plan, int axis, int choice);
 /* Default data pointers recorded at plan time (src/accfft_gpu.cpp:447-450). */
 int accfft_amd_plan_data(const accfft_amd_plan* plan, void** data, void** data_out);
 
