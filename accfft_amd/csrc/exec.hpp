@@ -50,8 +50,11 @@ inline int ensure_ws(accfft_amd_plan* p, int i, size_t bytes) {
 
 // Tiles per XCD chunk for a strided pass: rows whose stride is not a
 // multiple of 128 bytes share cache lines between neighbouring tiles, so
-// they go to one XCD in chunks of 16 (512^3 FP32 y pass 218 -> 205 us
-// against one contiguous range per XCD); aligned rows in chunks of 64.
+// they go to one XCD in chunks of 32 (512^3 FP32 y passes 216/220 us with
+// one contiguous range per XCD, 203/213 with chunks of 16, 201/211 with 32,
+// 209/223 with 64: smaller chunks re-read the lines shared across chunk
+// edges, larger ones spread the XCDs' concurrent tiles over more DRAM
+// pages); aligned rows in chunks of 64.
 // ACCFFT_AMD_XCD_GROUP_{MISALIGNED,ALIGNED} override (1 = round robin,
 // 0 = one contiguous range per XCD).
 inline int env_int(const char* name, int dflt) {
@@ -59,7 +62,7 @@ inline int env_int(const char* name, int dflt) {
   return e ? atoi(e) : dflt;
 }
 inline int xcd_group_for(int64_t row_stride_bytes) {
-  static const int g_mis = env_int("ACCFFT_AMD_XCD_GROUP_MISALIGNED", 16);
+  static const int g_mis = env_int("ACCFFT_AMD_XCD_GROUP_MISALIGNED", 32);
   static const int g_al = env_int("ACCFFT_AMD_XCD_GROUP_ALIGNED", 64);
   return (row_stride_bytes % 128) ? g_mis : g_al;
 }
