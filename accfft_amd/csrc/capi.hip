@@ -294,8 +294,13 @@ int accfft_amd_plan_set_tuning(accfft_amd_plan* p, int axis, int choice) {
                                             : afft::s4_config_ok<double>(n, choice);
     if (!ok || !has) return ACCFFT_AMD_ERR_INVALID;
   }
-  if (p->fp == ACCFFT_AMD_F32) p->tf[axis].s4v = choice;
-  else p->td[axis].s4v = choice;
+  if (p->fp == ACCFFT_AMD_F32) {
+    p->tf[axis].s4v = choice;
+    p->tf[axis].xg = -1;
+  } else {
+    p->td[axis].s4v = choice;
+    p->td[axis].xg = -1;
+  }
   return ACCFFT_AMD_OK;
 }
 

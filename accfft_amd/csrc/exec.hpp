@@ -66,6 +66,23 @@ inline int xcd_group_for(int64_t row_stride_bytes) {
   static const int g_al = env_int("ACCFFT_AMD_XCD_GROUP_ALIGNED", 64);
   return (row_stride_bytes % 128) ? g_mis : g_al;
 }
+// Non-temporal data loads / stores (FftArgs::nt) per pass kind, two bits
+// each (loads, stores): bits 0-1 z passes with real input (R2C, DERIV),
+// 2-3 z passes with complex input (C2R, C2C), 4-5 strided passes with
+// 128-byte aligned rows, 6-7 strided passes with misaligned rows.
+// Default: non-temporal loads for aligned strided rows, which the launch
+// keeps only when a tile's row segment covers whole 128-byte lines
+// (nt_gate): MI355X 512^3 FP32 x passes 225/217 -> 195/200 us, round trip
+// 1.33 -> 1.28 ms.  On 64-byte segments a line shared by two tiles is then
+// fetched twice (1024^3 x passes 2.07 -> 2.15 ms, slab y 2.75 -> 3.87 ms).
+// Non-temporal z loads speed the z C2R (199 -> 188 us) but the next
+// round trip's z R2C slows as much (200 -> 223 us); non-temporal stores
+// and misaligned rows lose everywhere.  ACCFFT_AMD_NT overrides.
+inline int nt_mode(bool strided, bool real_in, int64_t row_stride_bytes) {
+  static const int m = env_int("ACCFFT_AMD_NT", 1 << 4);
+  if (!strided) return real_in ? m & 3 : (m >> 2) & 3;
+  return (row_stride_bytes % 128) ? (m >> 6) & 3 : (m >> 4) & 3;
+}
 // Pass order: bit k of the mask reverses pass k of a 3-D transform
 // (forward: 0 z, 1 y, 2 x; inverse: 3 x, 4 y, 5 z).  A pass that starts
 // where the previous one ended reads what is still in the 256 MB
@@ -98,7 +115,11 @@ inline int run_pass(accfft_amd_plan* p, const char* tag, bool strided, int mode,
   if (order >= 0) a.reverse = reverse_pass(order) ? 1 : 0;
   if (strided) {
     const bool real_in = (mode == M_R2C || mode == M_DERIV);
-    a.xgroup = xcd_group_for(a.Sin * (int64_t)(real_in ? sizeof(T) : 2 * sizeof(T)));
+    const int64_t rs = a.Sin * (int64_t)(real_in ? sizeof(T) : 2 * sizeof(T));
+    a.xgroup = tab.xg >= 0 ? tab.xg : xcd_group_for(rs);
+    a.nt = nt_mode(true, real_in, rs);
+  } else {
+    a.nt = nt_mode(false, mode == M_R2C || mode == M_DERIV, 0);
   }
   auto launch = [&]() -> hipError_t {
     if constexpr (sizeof(T) == 4) {
@@ -550,9 +571,12 @@ struct Ops {
   // The reference's ACCFFT_MEASURE tunes its local FFT and transposes at
   // plan time (doc/flags.md).  Here, every strided C2C pass of the plan
   // whose length has several tile configurations times each candidate
-  // (one-level tiles, the two-level variants) on its real shape over a
-  // zeroed temporary buffer, and keeps the fastest when it beats the
-  // built-in rule by more than 3 %.  If the buffer cannot be allocated the
+  // (one-level tiles, the two-level variants, each with four XCD chunk
+  // sizes) on its real shape over a zeroed temporary buffer, and keeps the
+  // fastest when it beats the built-in rule by more than 3 %.  The best
+  // choice differs between MI355X machines (one box: 512^3 y passes
+  // 295/331 us with the two-level tiles, 229/241 with one-level tiles; the
+  // other boxes measured: 201/211 against 235).  If the buffer cannot be allocated the
   // rule stays.  As with FFTW's MEASURE, the choice can change results in
   // the last bits.
   static int tune(accfft_amd_plan* p) {
@@ -604,10 +628,11 @@ struct Ops {
       FftArgs<T> a = q.a;
       a.in = buf;
       a.out = q.out_elems ? (void*)((C*)buf + q.in_elems) : buf;
-      // mean of 5 launches after one warm-up; < 0 when the candidate fails
-      auto measure = [&](int cand) -> float {
+      // mean of 3 launches after one warm-up; < 0 when the candidate fails
+      auto measure = [&](int cand, int xg) -> float {
         t.s4v = cand;
-        for (int r = 0; r < 6; ++r) {
+        t.xg = xg;
+        for (int r = 0; r < 4; ++r) {
           if (r == 1) (void)hipEventRecord(e0, p->stream);
           if (run_pass<T>(p, "tune", true, M_C2C, -1, a, t, q.order)) return -1.f;
         }
@@ -616,31 +641,41 @@ struct Ops {
         if (hipEventSynchronize(e1) != hipSuccess ||
             hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
           return -1.f;
-        return ms / 5.f;
+        return ms / 3.f;
       };
-      // candidates: the rule (-1), one-level tiles (-2), the variants; two
-      // interleaved rounds, each candidate's best kept (clock ramps and
-      // neighbours' noise hit every candidate alike)
-      float best_of[6];
-      for (float& b : best_of) b = -1.f;
+      // candidates: tile configuration (the rule -1, one-level tiles -2, the
+      // variants) x tiles per XCD chunk (the rule -1, 16, 32, 64, one range
+      // per XCD 0); two interleaved rounds, each candidate's best kept
+      // (clock ramps and neighbours' noise hit every candidate alike)
+      constexpr int NG = 5;
+      const int groups[NG] = {-1, 16, 32, 64, 0};
+      float best_of[6][NG];
+      for (auto& row : best_of)
+        for (float& b : row) b = -1.f;
       for (int round = 0; round < 2; ++round)
         for (int cand = -1; cand < 4; cand = cand == -1 ? -2 : cand == -2 ? 0 : cand + 1) {
           if (cand >= 0 && !s4_config_ok<T>(t.n, cand)) continue;
-          const float ms = measure(cand);
-          float& b = best_of[cand + 2];
-          if (ms > 0.f && (b < 0.f || ms < b)) b = ms;
+          for (int gi = 0; gi < NG; ++gi) {
+            const float ms = measure(cand, groups[gi]);
+            float& b = best_of[cand + 2][gi];
+            if (ms > 0.f && (b < 0.f || ms < b)) b = ms;
+          }
         }
-      const float rule = best_of[1];
-      int best = -1;
+      const float rule = best_of[1][0];
+      int best = -1, best_g = -1;
       float best_ms = rule;
-      for (int cand = -2; cand < 4; ++cand) {
-        const float ms = best_of[cand + 2];
-        if (cand != -1 && ms > 0.f && rule > 0.f && ms < best_ms) {
-          best_ms = ms;
-          best = cand;
+      for (int cand = -2; cand < 4; ++cand)
+        for (int gi = 0; gi < NG; ++gi) {
+          const float ms = best_of[cand + 2][gi];
+          if (ms > 0.f && rule > 0.f && ms < best_ms) {
+            best_ms = ms;
+            best = cand;
+            best_g = groups[gi];
+          }
         }
-      }
-      t.s4v = (best != -1 && best_ms < 0.97f * rule) ? best : -1;
+      const bool take = best_ms < 0.97f * rule;
+      t.s4v = take ? best : -1;
+      t.xg = take ? best_g : -1;
     }
     (void)hipGetLastError();
     (void)hipStreamSynchronize(p->stream);
