@@ -66,23 +66,6 @@ inline int xcd_group_for(int64_t row_stride_bytes) {
   static const int g_al = env_int("ACCFFT_AMD_XCD_GROUP_ALIGNED", 64);
   return (row_stride_bytes % 128) ? g_mis : g_al;
 }
-// Non-temporal data loads / stores (FftArgs::nt) per pass kind, two bits
-// each (loads, stores): bits 0-1 z passes with real input (R2C, DERIV),
-// 2-3 z passes with complex input (C2R, C2C), 4-5 strided passes with
-// 128-byte aligned rows, 6-7 strided passes with misaligned rows.
-// Default: non-temporal loads for aligned strided rows, which the launch
-// keeps only when a tile's row segment covers whole 128-byte lines
-// (nt_gate): MI355X 512^3 FP32 x passes 225/217 -> 195/200 us, round trip
-// 1.33 -> 1.28 ms.  On 64-byte segments a line shared by two tiles is then
-// fetched twice (1024^3 x passes 2.07 -> 2.15 ms, slab y 2.75 -> 3.87 ms).
-// Non-temporal z loads speed the z C2R (199 -> 188 us) but the next
-// round trip's z R2C slows as much (200 -> 223 us); non-temporal stores
-// and misaligned rows lose everywhere.  ACCFFT_AMD_NT overrides.
-inline int nt_mode(bool strided, bool real_in, int64_t row_stride_bytes) {
-  static const int m = env_int("ACCFFT_AMD_NT", 1 << 4);
-  if (!strided) return real_in ? m & 3 : (m >> 2) & 3;
-  return (row_stride_bytes % 128) ? (m >> 6) & 3 : (m >> 4) & 3;
-}
 // Pass order: bit k of the mask reverses pass k of a 3-D transform
 // (forward: 0 z, 1 y, 2 x; inverse: 3 x, 4 y, 5 z).  A pass that starts
 // where the previous one ended reads what is still in the 256 MB
@@ -117,9 +100,6 @@ inline int run_pass(accfft_amd_plan* p, const char* tag, bool strided, int mode,
     const bool real_in = (mode == M_R2C || mode == M_DERIV);
     const int64_t rs = a.Sin * (int64_t)(real_in ? sizeof(T) : 2 * sizeof(T));
     a.xgroup = tab.xg >= 0 ? tab.xg : xcd_group_for(rs);
-    a.nt = nt_mode(true, real_in, rs);
-  } else {
-    a.nt = nt_mode(false, mode == M_R2C || mode == M_DERIV, 0);
   }
   auto launch = [&]() -> hipError_t {
     if constexpr (sizeof(T) == 4) {
@@ -573,10 +553,10 @@ struct Ops {
   // whose length has several tile configurations times each candidate
   // (one-level tiles, the two-level variants, each with four XCD chunk
   // sizes) on its real shape over a zeroed temporary buffer, and keeps the
-  // fastest when it beats the built-in rule by more than 3 %.  The best
-  // choice differs between MI355X machines (one box: 512^3 y passes
-  // 295/331 us with the two-level tiles, 229/241 with one-level tiles; the
-  // other boxes measured: 201/211 against 235).  If the buffer cannot be allocated the
+  // fastest when it beats the built-in rule by more than 3 %.  It guards
+  // against shapes and machines where the rule loses: with a slower build
+  // of the two-level tiles (512^3 y passes 285/315 us) it picked the
+  // one-level tiles (236/240 us) on its own.  If the buffer cannot be allocated the
   // rule stays.  As with FFTW's MEASURE, the choice can change results in
   // the last bits.
   static int tune(accfft_amd_plan* p) {

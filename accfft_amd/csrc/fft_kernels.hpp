@@ -51,7 +51,6 @@ struct FftArgs {
   // This is the pack / unpack of a slab exchange fused into the y pass.
   int ishift, oshift;
   int64_t Bin, Bout;
-  int nt;  // bit 0: non-temporal loads, bit 1: non-temporal stores of the data
 };
 
 __device__ __forceinline__ int64_t row_off(int r, int64_t S, int shift, int64_t B) {
@@ -78,50 +77,37 @@ struct SCfg {
   static constexpr int NT = TPB / W;
 };
 
-// Element loads and stores of the data; nt (uniform) streams them past the
-// caches (non-temporal): bit 0 loads, bit 1 stores.
 template <typename T>
-struct Vec2 {
-  typedef T type __attribute__((ext_vector_type(2)));
-};
-template <typename T>
-__device__ __forceinline__ cpx<T> ldp(const cpx<T>* q, int nt) {
-  if (nt & 1) {
-    const typename Vec2<T>::type v =
-        __builtin_nontemporal_load(reinterpret_cast<const typename Vec2<T>::type*>(q));
-    return {v.x, v.y};
-  }
-  return *q;
+__device__ __forceinline__ cpx<T> ldc(const void* p, int64_t i) {
+  return reinterpret_cast<const cpx<T>*>(p)[i];
 }
 template <typename T>
-__device__ __forceinline__ void stp(cpx<T>* q, cpx<T> v, int nt) {
-  if (nt & 2) {
-    typename Vec2<T>::type w;
-    w.x = v.x;
-    w.y = v.y;
-    __builtin_nontemporal_store(w, reinterpret_cast<typename Vec2<T>::type*>(q));
-    return;
-  }
-  *q = v;
+__device__ __forceinline__ T ldr(const void* p, int64_t i) {
+  return reinterpret_cast<const T*>(p)[i];
+}
+// Non-temporal result stores (AFFT_NT_STORES=1) were measured slower on
+// MI355X: the y pass of the r2c layout writes rows that straddle 128-byte
+// lines, and partial-line non-temporal writes cannot merge in L2.
+#ifndef AFFT_NT_STORES
+#define AFFT_NT_STORES 0
+#endif
+template <typename T>
+__device__ __forceinline__ void stc(void* p, int64_t i, cpx<T> v) {
+#if AFFT_NT_STORES
+  T* q = reinterpret_cast<T*>(p) + 2 * i;
+  __builtin_nontemporal_store(v.x, q);
+  __builtin_nontemporal_store(v.y, q + 1);
+#else
+  reinterpret_cast<cpx<T>*>(p)[i] = v;
+#endif
 }
 template <typename T>
-__device__ __forceinline__ cpx<T> ldc(const void* p, int64_t i, int nt = 0) {
-  return ldp(reinterpret_cast<const cpx<T>*>(p) + i, nt);
-}
-template <typename T>
-__device__ __forceinline__ T ldr(const void* p, int64_t i, int nt = 0) {
-  const T* q = reinterpret_cast<const T*>(p) + i;
-  return (nt & 1) ? __builtin_nontemporal_load(q) : *q;
-}
-template <typename T>
-__device__ __forceinline__ void stc(void* p, int64_t i, cpx<T> v, int nt = 0) {
-  stp(reinterpret_cast<cpx<T>*>(p) + i, v, nt);
-}
-template <typename T>
-__device__ __forceinline__ void str(void* p, int64_t i, T v, int nt = 0) {
-  T* q = reinterpret_cast<T*>(p) + i;
-  if (nt & 2) __builtin_nontemporal_store(v, q);
-  else *q = v;
+__device__ __forceinline__ void str(void* p, int64_t i, T v) {
+#if AFFT_NT_STORES
+  __builtin_nontemporal_store(v, reinterpret_cast<T*>(p) + i);
+#else
+  reinterpret_cast<T*>(p)[i] = v;
+#endif
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs of an MI355X.  This
@@ -253,8 +239,8 @@ __global__ void __launch_bounds__(256) zfft_kernel(FftArgs<T> a) {
         if (q < NHT) {
           const int p = q / NH, k = q % NH;
           const int64_t ga = 2 * (l0 + p), gb = ga + 1;
-          if (ga < nl) A[j] = ldc<T>(a.in, ga * a.idist + k, a.nt);
-          if (gb < nl) B[j] = ldc<T>(a.in, gb * a.idist + k, a.nt);
+          if (ga < nl) A[j] = ldc<T>(a.in, ga * a.idist + k);
+          if (gb < nl) B[j] = ldc<T>(a.in, gb * a.idist + k);
         }
       }
       stage_twiddles<T, TWN, TPB>(const_cast<cpx<T>*>(twl), a.tw, t);
@@ -283,11 +269,11 @@ __global__ void __launch_bounds__(256) zfft_kernel(FftArgs<T> a) {
       const int p = e / NS_, i = e % NS_;
       if constexpr (MODE == M_C2C || MODE == M_MULT) {
         const int64_t g = l0 + p;
-        v[j] = g < nl ? ldc<T>(a.in, g * a.idist + i, a.nt) : cpx<T>{(T)0, (T)0};
+        v[j] = g < nl ? ldc<T>(a.in, g * a.idist + i) : cpx<T>{(T)0, (T)0};
       } else if constexpr (MODE == M_R2C || MODE == M_DERIV) {
         const int64_t ra = 2 * (l0 + p), rb = ra + 1;
-        v[j].x = ra < nl ? ldr<T>(a.in, ra * a.idist + i, a.nt) : (T)0;
-        v[j].y = rb < nl ? ldr<T>(a.in, rb * a.idist + i, a.nt) : (T)0;
+        v[j].x = ra < nl ? ldr<T>(a.in, ra * a.idist + i) : (T)0;
+        v[j].y = rb < nl ? ldr<T>(a.in, rb * a.idist + i) : (T)0;
       }
     }
     stage_twiddles<T, TWN, TPB>(const_cast<cpx<T>*>(twl), a.tw, t);
@@ -299,7 +285,7 @@ __global__ void __launch_bounds__(256) zfft_kernel(FftArgs<T> a) {
       const int p = e / n, i = e - p * n;
       const int64_t g = l0 + p;
       cpx<T> v = {(T)0, (T)0};
-      if (g < nl) v = ldc<T>(a.in, g * a.idist + i, a.nt);
+      if (g < nl) v = ldc<T>(a.in, g * a.idist + i);
       bufA[e] = v;
     }
   } else if constexpr (MODE == M_R2C || MODE == M_DERIV) {
@@ -307,8 +293,8 @@ __global__ void __launch_bounds__(256) zfft_kernel(FftArgs<T> a) {
       const int p = e / n, i = e - p * n;
       const int64_t ra = 2 * (l0 + p), rb = ra + 1;
       cpx<T> v = {(T)0, (T)0};
-      if (ra < nl) v.x = ldr<T>(a.in, ra * a.idist + i, a.nt);
-      if (rb < nl) v.y = ldr<T>(a.in, rb * a.idist + i, a.nt);
+      if (ra < nl) v.x = ldr<T>(a.in, ra * a.idist + i);
+      if (rb < nl) v.y = ldr<T>(a.in, rb * a.idist + i);
       bufA[e] = v;
     }
   } else {  // M_C2R
@@ -317,8 +303,8 @@ __global__ void __launch_bounds__(256) zfft_kernel(FftArgs<T> a) {
       const int64_t ga = 2 * (l0 + p), gb = ga + 1;
       const int k = i < nh ? i : n - i;
       cpx<T> A = {(T)0, (T)0}, B = {(T)0, (T)0};
-      if (ga < nl) A = ldc<T>(a.in, ga * a.idist + k, a.nt);
-      if (gb < nl) B = ldc<T>(a.in, gb * a.idist + k, a.nt);
+      if (ga < nl) A = ldc<T>(a.in, ga * a.idist + k);
+      if (gb < nl) B = ldc<T>(a.in, gb * a.idist + k);
       if (k == 0 || 2 * k == n) {
         A.y = (T)0;
         B.y = (T)0;
@@ -369,7 +355,7 @@ __global__ void __launch_bounds__(256) zfft_kernel(FftArgs<T> a) {
       if (!STATIC && e >= lpw * n) break;
       const int p = e / n, i = e - p * n;
       const int64_t g = l0 + p;
-      if (g < nl) stc<T>(a.out, g * a.odist + i, res[e], a.nt);
+      if (g < nl) stc<T>(a.out, g * a.odist + i, res[e]);
     }
   } else if constexpr (MODE == M_R2C) {
 #pragma unroll
@@ -387,7 +373,7 @@ __global__ void __launch_bounds__(256) zfft_kernel(FftArgs<T> a) {
       if ((rl & 1) == 0) o = {h * (z1.x + z2.x), h * (z1.y - z2.y)};
       else o = {h * (z1.y + z2.y), h * (z2.x - z1.x)};
       if (a.adjw && k != 0 && 2 * k != n) o = cscale(o, (T)2);
-      stc<T>(a.out, g * a.odist + k, o, a.nt);
+      stc<T>(a.out, g * a.odist + k, o);
     }
   } else {  // real outputs
 #pragma unroll
@@ -404,8 +390,8 @@ __global__ void __launch_bounds__(256) zfft_kernel(FftArgs<T> a) {
           continue;
         }
       }
-      if (ra < nl) str<T>(a.out, ra * a.odist + i, z.x, a.nt);
-      if (rb < nl) str<T>(a.out, rb * a.odist + i, z.y, a.nt);
+      if (ra < nl) str<T>(a.out, ra * a.odist + i, z.x);
+      if (rb < nl) str<T>(a.out, rb * a.odist + i, z.y);
     }
   }
 }
@@ -472,14 +458,14 @@ __global__ void __launch_bounds__(256) sfft_kernel(FftArgs<T> a) {
     for (int j = 0; j < RP; ++j) {
       const int r = r0 + j * RS;
       if constexpr (MODE == M_C2C || MODE == M_MULT) {
-        v[j] = va ? ldc<T>(a.in, ia + row_off(r, a.Sin, a.ishift, a.Bin), a.nt) : cpx<T>{(T)0, (T)0};
+        v[j] = va ? ldc<T>(a.in, ia + row_off(r, a.Sin, a.ishift, a.Bin)) : cpx<T>{(T)0, (T)0};
       } else if constexpr (MODE == M_R2C || MODE == M_DERIV) {
-        v[j].x = va ? ldr<T>(a.in, ia + (int64_t)r * a.Sin, a.nt) : (T)0;
-        v[j].y = vb ? ldr<T>(a.in, ib + (int64_t)r * a.Sin, a.nt) : (T)0;
+        v[j].x = va ? ldr<T>(a.in, ia + (int64_t)r * a.Sin) : (T)0;
+        v[j].y = vb ? ldr<T>(a.in, ib + (int64_t)r * a.Sin) : (T)0;
       } else {
         const int k = r < NH ? r : NS_ - r;
-        cpx<T> A = va ? ldc<T>(a.in, ia + (int64_t)k * a.Sin, a.nt) : cpx<T>{(T)0, (T)0};
-        cpx<T> B = vb ? ldc<T>(a.in, ib + (int64_t)k * a.Sin, a.nt) : cpx<T>{(T)0, (T)0};
+        cpx<T> A = va ? ldc<T>(a.in, ia + (int64_t)k * a.Sin) : cpx<T>{(T)0, (T)0};
+        cpx<T> B = vb ? ldc<T>(a.in, ib + (int64_t)k * a.Sin) : cpx<T>{(T)0, (T)0};
         if (k == 0 || 2 * k == NS_) { A.y = (T)0; B.y = (T)0; }
         v[j] = r < NH ? cpx<T>{A.x - B.y, A.y + B.x} : cpx<T>{A.x + B.y, B.x - A.y};
       }
@@ -490,22 +476,22 @@ __global__ void __launch_bounds__(256) sfft_kernel(FftArgs<T> a) {
   } else if constexpr (MODE == M_C2C || MODE == M_MULT) {
     for (int r = r0; r < n; r += rstep) {
       cpx<T> v = {(T)0, (T)0};
-      if (va) v = ldc<T>(a.in, ia + row_off(r, a.Sin, a.ishift, a.Bin), a.nt);
+      if (va) v = ldc<T>(a.in, ia + row_off(r, a.Sin, a.ishift, a.Bin));
       bufA[r * W + w] = v;
     }
   } else if constexpr (MODE == M_R2C || MODE == M_DERIV) {
     for (int r = r0; r < n; r += rstep) {
       cpx<T> v = {(T)0, (T)0};
-      if (va) v.x = ldr<T>(a.in, ia + (int64_t)r * a.Sin, a.nt);
-      if (vb) v.y = ldr<T>(a.in, ib + (int64_t)r * a.Sin, a.nt);
+      if (va) v.x = ldr<T>(a.in, ia + (int64_t)r * a.Sin);
+      if (vb) v.y = ldr<T>(a.in, ib + (int64_t)r * a.Sin);
       bufA[r * W + w] = v;
     }
   } else {  // M_C2R
     for (int r = r0; r < n; r += rstep) {
       const int k = r < nh ? r : n - r;
       cpx<T> A = {(T)0, (T)0}, B = {(T)0, (T)0};
-      if (va) A = ldc<T>(a.in, ia + (int64_t)k * a.Sin, a.nt);
-      if (vb) B = ldc<T>(a.in, ib + (int64_t)k * a.Sin, a.nt);
+      if (va) A = ldc<T>(a.in, ia + (int64_t)k * a.Sin);
+      if (vb) B = ldc<T>(a.in, ib + (int64_t)k * a.Sin);
       if (k == 0 || 2 * k == n) { A.y = (T)0; B.y = (T)0; }
       cpx<T> z;
       if (r < nh) z = {A.x - B.y, A.y + B.x};
@@ -551,7 +537,7 @@ __global__ void __launch_bounds__(256) sfft_kernel(FftArgs<T> a) {
 #pragma unroll
       for (int j = 0; j < rp; ++j) {
         const int r = r0 + j * rstep;
-        stc<T>(a.out, oa + row_off(r, a.Sout, a.oshift, a.Bout), res[r * W + w], a.nt);
+        stc<T>(a.out, oa + row_off(r, a.Sout, a.oshift, a.Bout), res[r * W + w]);
       }
     }
   } else if constexpr (MODE == M_R2C) {
@@ -563,8 +549,8 @@ __global__ void __launch_bounds__(256) sfft_kernel(FftArgs<T> a) {
       if (k >= nh) break;
       const cpx<T> z1 = res[k * W + w];
       const cpx<T> z2 = res[(k == 0 ? 0 : n - k) * W + w];
-      if (va) stc<T>(a.out, oa + (int64_t)k * a.Sout, cpx<T>{h * (z1.x + z2.x), h * (z1.y - z2.y)}, a.nt);
-      if (vb) stc<T>(a.out, ob + (int64_t)k * a.Sout, cpx<T>{h * (z1.y + z2.y), h * (z2.x - z1.x)}, a.nt);
+      if (va) stc<T>(a.out, oa + (int64_t)k * a.Sout, cpx<T>{h * (z1.x + z2.x), h * (z1.y - z2.y)});
+      if (vb) stc<T>(a.out, ob + (int64_t)k * a.Sout, cpx<T>{h * (z1.y + z2.y), h * (z2.x - z1.x)});
     }
   } else {
 #pragma unroll
@@ -578,8 +564,8 @@ __global__ void __launch_bounds__(256) sfft_kernel(FftArgs<T> a) {
           continue;
         }
       }
-      if (va) str<T>(a.out, oa + (int64_t)r * a.Sout, z.x, a.nt);
-      if (vb) str<T>(a.out, ob + (int64_t)r * a.Sout, z.y, a.nt);
+      if (va) str<T>(a.out, oa + (int64_t)r * a.Sout, z.x);
+      if (vb) str<T>(a.out, ob + (int64_t)r * a.Sout, z.y);
     }
   }
 }
@@ -668,7 +654,7 @@ struct S4Tile {
 #pragma unroll
         for (int n1 = 0; n1 < R1; ++n1) {
           const int row = r0 + d * RS1 + M * n1;
-          v[d][n1] = ok ? ldp(src + row_off(row, a.Sin, a.ishift, a.Bin), a.nt) : cpx<T>{(T)0, (T)0};
+          v[d][n1] = ok ? src[row_off(row, a.Sin, a.ishift, a.Bin)] : cpx<T>{(T)0, (T)0};
         }
       } else {
         // running pointer, opaque to the compiler: the R1 row offsets are
@@ -677,7 +663,7 @@ struct S4Tile {
         const int64_t step = (int64_t)M * a.Sin;
 #pragma unroll
         for (int n1 = 0; n1 < R1; ++n1) {
-          v[d][n1] = ok ? ldp(p, a.nt) : cpx<T>{(T)0, (T)0};
+          v[d][n1] = ok ? *p : cpx<T>{(T)0, (T)0};
           p += step;
           asm volatile("" : "+v"(p));
         }
@@ -799,7 +785,7 @@ struct S4Tile {
 #pragma unroll
           for (int k2 = r0; k2 < M; k2 += RS1) {
             const int row = kc0 + kk + R1 * k2;
-            stp(dst + row_off(row, a.Sout, a.oshift, a.Bout), buf[(kk * M + k2) * W + c], a.nt);
+            dst[row_off(row, a.Sout, a.oshift, a.Bout)] = buf[(kk * M + k2) * W + c];
           }
       } else {
         // running pointers (rows kc0 + kk + R1 * k2), opaque to the
@@ -811,7 +797,7 @@ struct S4Tile {
           cpx<T>* p = p0;
 #pragma unroll
           for (int k2 = r0; k2 < M; k2 += RS1) {
-            stp(p, buf[(kk * M + k2) * W + c], a.nt);
+            *p = buf[(kk * M + k2) * W + c];
             p += step;
             asm volatile("" : "+v"(p));
           }

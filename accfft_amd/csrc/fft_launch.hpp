@@ -226,13 +226,6 @@ inline int device_cus() {
   return cus;
 }
 
-// Non-temporal loads only where a tile's row segment (W elements) covers
-// whole 128-byte lines: a line shared by two tiles would be fetched twice.
-template <typename T>
-inline void nt_gate(FftArgs<T>& a, int W) {
-  if ((int64_t)W * (int64_t)sizeof(cpx<T>) < 128) a.nt &= ~1;
-}
-
 template <typename T, int N, int V, int SPLIT = 0>
 hipError_t launch_s4v(int sign, FftArgs<T>& a, const AxisTab<T>& tab, hipStream_t st) {
   if constexpr (!S4Cfg<T, N, V>::OK) {
@@ -242,7 +235,6 @@ hipError_t launch_s4v(int sign, FftArgs<T>& a, const AxisTab<T>& tab, hipStream_
     const int64_t tiles = (a.nlines + CF::W - 1) / CF::W;
     a.tw = tab.tw_full;
     a.lpw = CF::W;
-    nt_gate(a, CF::W);
     int64_t grid = tiles;
     if constexpr (CF::PB > 0) {
       // persistent: PB workgroups per CU, a multiple of 8 (one per XCD)
@@ -301,7 +293,6 @@ hipError_t launch_fft(bool strided, int mode, int sign, FftArgs<T> a, const Axis
       const int w = n == 1024 ? SCfg<T, 1024, 0, 128>::W
                   : n == 2048 ? SCfg<T, 2048, 0, 128>::W : SCfg<T, 4096, 0, 128>::W;
       a.lpw = w;
-      nt_gate(a, w);
       const int64_t grid = (units + w - 1) / w;
       dim3 g((unsigned)grid), b(256);
 #define AFFT_WIDE(NN)                                                                     \
@@ -320,7 +311,6 @@ hipError_t launch_fft(bool strided, int mode, int sign, FftArgs<T> a, const Axis
       const int w = n == 512 ? SCfg<T, 512, 0, 32>::W
                   : n == 1024 ? SCfg<T, 1024, 0, 32>::W : SCfg<T, 2048, 0, 32>::W;
       a.lpw = w;
-      nt_gate(a, w);
       const int64_t grid = (units + w - 1) / w;
       dim3 g((unsigned)grid), b(256);
 #define AFFT_NARROW(NN)                                                                  \
@@ -338,7 +328,6 @@ hipError_t launch_fft(bool strided, int mode, int sign, FftArgs<T> a, const Axis
   case NN:                                                                         \
     per = strided ? SCfg<T, NN>::W : ZCfg<NN>::LPW;                                \
     a.lpw = per;                                                                   \
-    if (strided) nt_gate(a, per);                                                  \
     return strided ? launch_s_n<T, NN>(mode, sign, a, (units + per - 1) / per, 0, st) \
                    : launch_z_n<T, NN>(mode, sign, a, (units + per - 1) / per, 0, st);
     switch (n) {
@@ -362,7 +351,6 @@ hipError_t launch_fft(bool strided, int mode, int sign, FftArgs<T> a, const Axis
     const size_t shm = dyn_s_lds(n, w, csz);
     if (shm > 160 * 1024) return hipErrorNotSupported;
     a.lpw = w;
-    nt_gate(a, w);
     return launch_s_n<T, 0>(mode, sign, a, (units + w - 1) / w, shm, st);
   } else {
     const int l = dyn_lines(n);

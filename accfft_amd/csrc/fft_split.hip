@@ -11,7 +11,6 @@ static hipError_t launch_split_n(bool strided, int mode, int sign, FftArgs<float
                                  int64_t units, hipStream_t st) {
   const int per = strided ? SCfg<float, N, SPLIT>::W : ZCfg<N, SPLIT>::LPW;
   a.lpw = per;
-  if (strided) nt_gate(a, per);
   dim3 g((unsigned)((units + per - 1) / per)), b(256);
   if (strided) {
     if (mode != M_C2C) return hipErrorInvalidValue;
