@@ -61,6 +61,15 @@ inline int env_int(const char* name, int dflt) {
   const char* e = getenv(name);
   return e ? atoi(e) : dflt;
 }
+// Blocks per XCD chunk of a z pass: consecutive blocks read and write
+// adjacent rows (2056-byte rows of the r2c half spectrum share lines at
+// block edges), so chunks keep those edges in one L2.  512^3 FP32 z R2C /
+// C2R 196/207 us round robin, 183/201 with chunks of 16, 180/197 with 32,
+// 188/199 with 128.  ACCFFT_AMD_XCD_GROUP_Z overrides; 1 = round robin.
+inline int xcd_group_z() {
+  static const int g = env_int("ACCFFT_AMD_XCD_GROUP_Z", 32);
+  return g;
+}
 inline int xcd_group_for(int64_t row_stride_bytes) {
   static const int g_mis = env_int("ACCFFT_AMD_XCD_GROUP_MISALIGNED", 32);
   static const int g_al = env_int("ACCFFT_AMD_XCD_GROUP_ALIGNED", 64);
@@ -100,6 +109,8 @@ inline int run_pass(accfft_amd_plan* p, const char* tag, bool strided, int mode,
     const bool real_in = (mode == M_R2C || mode == M_DERIV);
     const int64_t rs = a.Sin * (int64_t)(real_in ? sizeof(T) : 2 * sizeof(T));
     a.xgroup = tab.xg >= 0 ? tab.xg : xcd_group_for(rs);
+  } else {
+    a.xgroup = xcd_group_z();
   }
   auto launch = [&]() -> hipError_t {
     if constexpr (sizeof(T) == 4) {

@@ -215,7 +215,8 @@ __global__ void __launch_bounds__(256) zfft_kernel(FftArgs<T> a) {
 
   // reverse: walk the lines last-to-first, so a pass can begin with the data
   // its predecessor wrote last (still in the memory-side cache)
-  const int64_t blk = a.reverse ? (int64_t)gridDim.x - 1 - blockIdx.x : (int64_t)blockIdx.x;
+  const int64_t bx = xcd_tile(blockIdx.x, gridDim.x, (unsigned)a.xgroup);
+  const int64_t blk = a.reverse ? (int64_t)gridDim.x - 1 - bx : bx;
   const int64_t l0 = blk * lpw;  // first complex line / pair
   const int64_t nl = a.nlines;
 
