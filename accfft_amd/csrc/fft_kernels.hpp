@@ -20,6 +20,8 @@
 //           (the per-axis r2c / grad_mult / c2r chain of
 //           operators_gpu.txx:172-278 in one pass, optional accumulate)
 #pragma once
+#include <type_traits>
+
 #include "fft_mfma.hpp"
 
 namespace afft {
@@ -895,6 +897,180 @@ __global__ void __launch_bounds__((S4Cfg<T, N, V>::TPB), (S4Cfg<T, N, V>::WPE))
       if (!more) break;
       vb = nvb;
       tile = ntile;
+    }
+  }
+}
+
+// ====================================================================
+// Fused two-level tiles: forward FFT, spectral symbol, inverse FFT
+// ====================================================================
+// M_MULT (a 3-D symbol, the x pass of laplace and friends) and M_DERIV
+// (i w(k) / N along this axis, on a real field viewed as complex pairs of
+// adjacent columns) with the whole tile of W columns x N rows resident:
+//   forward: x[n2 + M n1] -> registers, R1-point DFTs and twiddles, LDS,
+//            M-point DFTs over n2 -> X[k1 + R1 k2] at (k1, k2) in LDS;
+//   symbol:  in LDS;
+//   inverse: M-point inverse DFTs over k2 -> Y[k1][n2], twiddles
+//            exp(+2 pi i k1 n2 / N) and R1-point inverse DFTs over k1 in
+//            registers -> x'[n2 + M n1] stored straight to HBM
+// (x'[n2 + M n1] = sum_k1 e^{2 pi i k1 n1 / R1} e^{2 pi i k1 n2 / N}
+//  sum_k2 X[k1 + R1 k2] e^{2 pi i k2 n2 / M}).  One read and one write of
+// the operand, like the one-level fused tile, but without its LDS
+// round trips of the first and last levels.
+template <typename T, int N>
+struct S4FCfg {
+  static constexpr bool OK = false;
+  static constexpr int W = 1, TPB = 256;
+};
+#define AFFT_S4FCFG(TT, NN, W_, TPB_)                   \
+  template <>                                           \
+  struct S4FCfg<TT, NN> {                               \
+    static constexpr bool OK = true;                    \
+    static constexpr int W = W_, TPB = TPB_;            \
+  };
+AFFT_S4FCFG(float, 512, 16, 512)
+AFFT_S4FCFG(float, 1024, 8, 512)
+AFFT_S4FCFG(float, 2048, 4, 512)
+#undef AFFT_S4FCFG
+
+template <typename T, int N, int MODE>
+__global__ void __launch_bounds__((S4FCfg<T, N>::TPB))
+    sfft4f_kernel(FftArgs<T> a, const cpx<T>* __restrict__ twsub) {
+  using CF = S4FCfg<T, N>;
+  constexpr int R1 = 16, W = CF::W, TPB = CF::TPB, M = N / R1;
+  constexpr int D1 = W * M / TPB, RS1 = TPB / W, NT = TPB / (R1 * W);
+  constexpr int TWM = sched_twn<M, 0>();
+  static_assert(MODE == M_MULT || MODE == M_DERIV, "fused modes only");
+  static_assert(D1 >= 1 && D1 * TPB == W * M, "tile must split evenly");
+  static_assert(NT >= 1 && NT * R1 * W == TPB, "lines must split evenly");
+  __shared__ cpx<T> buf[R1 * M * W];
+  __shared__ cpx<T> tw1[N];
+  __shared__ cpx<T> tw2[TWM > 0 ? TWM : 1];
+  const int t = threadIdx.x;
+  const int c = t % W, r0 = t / W;
+  int64_t tile = xcd_tile(blockIdx.x, gridDim.x, (unsigned)a.xgroup);
+  if (a.reverse) tile = (int64_t)gridDim.x - 1 - tile;
+  const int64_t col = tile * W + c;
+  const bool ok = col < a.nlines;
+  const int64_t ib = (col / a.C) * a.Din + (col % a.C);
+  const int64_t ob = (col / a.C) * a.Dout + (col % a.C);
+
+  constexpr int E1 = (N + TPB - 1) / TPB, E2 = (TWM + TPB - 1) / TPB;
+  cpx<T> tv1[E1], tv2[E2 > 0 ? E2 : 1];
+#pragma unroll
+  for (int j = 0; j < E1; ++j)
+    if (t + j * TPB < N) tv1[j] = a.tw[t + j * TPB];
+#pragma unroll
+  for (int j = 0; j < E2; ++j)
+    if (t + j * TPB < TWM) tv2[j] = twsub[t + j * TPB];
+  cpx<T> v[D1][R1];
+  {
+    const cpx<T>* src = reinterpret_cast<const cpx<T>*>(a.in) + ib;
+#pragma unroll
+    for (int d = 0; d < D1; ++d) {
+      const cpx<T>* p = src + (int64_t)(r0 + d * RS1) * a.Sin;
+      const int64_t step = (int64_t)M * a.Sin;
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) {
+        v[d][n1] = ok ? *p : cpx<T>{(T)0, (T)0};
+        p += step;
+        asm volatile("" : "+v"(p));
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < E1; ++j)
+    if (t + j * TPB < N) tw1[t + j * TPB] = tv1[j];
+#pragma unroll
+  for (int j = 0; j < E2; ++j)
+    if (t + j * TPB < TWM) tw2[t + j * TPB] = tv2[j];
+  __syncthreads();
+
+  // forward first level, then the whole tile to LDS
+#pragma unroll
+  for (int d = 0; d < D1; ++d) {
+    dft<T, R1, -1>(v[d]);
+    const int n2 = r0 + d * RS1;
+#pragma unroll
+    for (int k1 = 1; k1 < R1; ++k1) v[d][k1] = cmul(v[d][k1], tw1[n2 * k1]);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) buf[(k1 * M + n2) * W + c] = v[d][k1];
+  }
+  __syncthreads();
+  const int q = r0 % NT, kl = r0 / NT;  // line (k1 = kl, column c), thread q of NT
+  LineAcc<T, W> acc{buf + kl * M * W + c};
+  static_fft<T, M, NT, -1, W, 0, 1>(acc, q, tw2, buf, 0);
+
+  // symbol on X[k1 + R1 k2] at (k1, k2); thread t's elements all lie in
+  // its own column c (TPB is a multiple of W)
+  {
+    int64_t u = 0, Z = 0;
+    if constexpr (MODE == M_MULT) {
+      u = col / a.Ch;
+      Z = col % a.Ch;
+    }
+    // one loop per symbol kind (the kind is uniform: the switch stays out
+    // of the loop and apply_sym folds to that kind's arithmetic)
+    auto sym_loop = [&](auto kind) {
+#pragma unroll
+      for (int i = 0; i < R1 * D1; ++i) {
+        const int e = t + i * TPB;
+        const int k2 = (e / W) % M, k1 = e / (M * W);
+        const long k = (long)(k1 + R1 * k2);
+        const cpx<T> x = buf[e];
+        if constexpr (MODE == M_DERIV) {
+          const T f = (T)(a.scale * (double)wave_sym(k, N));
+          buf[e] = cpx<T>{-f * x.y, f * x.x};
+        } else {
+          const long X = a.maxis == 0 ? k : (long)u, Y = a.maxis == 0 ? (long)u : k;
+          buf[e] = apply_sym<T>(decltype(kind)::value, X, Y, (long)Z, a.N0, a.N1, a.N2, a.scale, x);
+        }
+      }
+    };
+    if constexpr (MODE == M_DERIV) {
+      sym_loop(std::integral_constant<int, 0>{});
+    } else {
+      switch (a.sym) {
+        case SYM_GRAD_X: sym_loop(std::integral_constant<int, SYM_GRAD_X>{}); break;
+        case SYM_GRAD_Y: sym_loop(std::integral_constant<int, SYM_GRAD_Y>{}); break;
+        case SYM_GRAD_Z: sym_loop(std::integral_constant<int, SYM_GRAD_Z>{}); break;
+        case SYM_LAPLACE: sym_loop(std::integral_constant<int, SYM_LAPLACE>{}); break;
+        case SYM_BIHARMONIC: sym_loop(std::integral_constant<int, SYM_BIHARMONIC>{}); break;
+        case SYM_INV_LAPLACE: sym_loop(std::integral_constant<int, SYM_INV_LAPLACE>{}); break;
+        case SYM_INV_BIHARMONIC: sym_loop(std::integral_constant<int, SYM_INV_BIHARMONIC>{}); break;
+        case SYM_POISSON_DISCRETE: sym_loop(std::integral_constant<int, SYM_POISSON_DISCRETE>{}); break;
+        default: sym_loop(std::integral_constant<int, SYM_POISSON_CONTINUOUS>{}); break;
+      }
+    }
+  }
+  __syncthreads();
+  static_fft<T, M, NT, +1, W, 0, 1>(acc, q, tw2, buf, 0);
+
+  // inverse first level: twiddles, R1-point inverse DFTs, store
+  cpx<T>* dst = reinterpret_cast<cpx<T>*>(a.out) + ob;
+#pragma unroll
+  for (int d = 0; d < D1; ++d) {
+    const int n2 = r0 + d * RS1;
+    cpx<T> w[R1];
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) w[k1] = buf[(k1 * M + n2) * W + c];
+#pragma unroll
+    for (int k1 = 1; k1 < R1; ++k1) w[k1] = cmulc(w[k1], tw1[n2 * k1]);
+    dft<T, R1, +1>(w);
+    if (ok) {
+      cpx<T>* p = dst + (int64_t)n2 * a.Sout;
+      const int64_t step = (int64_t)M * a.Sout;
+#pragma unroll
+      for (int n1 = 0; n1 < R1; ++n1) {
+        if (MODE == M_DERIV && a.accumulate) {
+          const cpx<T> o = *p;
+          *p = cpx<T>{o.x + w[n1].x, o.y + w[n1].y};
+        } else {
+          *p = w[n1];
+        }
+        p += step;
+        asm volatile("" : "+v"(p));
+      }
     }
   }
 }

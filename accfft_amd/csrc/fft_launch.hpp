@@ -267,6 +267,41 @@ hipError_t launch_s4(int sign, FftArgs<T>& a, const AxisTab<T>& tab, hipStream_t
   }
 }
 
+// Fused two-level tiles for a strided M_MULT / M_DERIV pass (FP32 512,
+// 1024, 2048); hipErrorNotSupported sends the caller to the one-level tile.
+// M_DERIV runs on the complex view of adjacent real columns, which needs
+// even column counts, strides and 8-byte aligned pointers.
+template <typename T>
+hipError_t launch_s4f(int mode, FftArgs<T> a, const AxisTab<T>& tab, hipStream_t st) {
+  if (mode == M_DERIV) {
+    if ((a.nlines | a.C | a.Din | a.Dout | a.Sin | a.Sout) & 1) return hipErrorNotSupported;
+    if (((uintptr_t)a.in | (uintptr_t)a.out) % sizeof(cpx<T>)) return hipErrorNotSupported;
+    a.nlines /= 2;
+    a.C /= 2;
+    a.Din /= 2;
+    a.Dout /= 2;
+    a.Sin /= 2;
+    a.Sout /= 2;
+  }
+  a.tw = tab.tw_full;
+#define AFFT_S4F(NN)                                                                        \
+  if (tab.n == NN) {                                                                        \
+    if constexpr (S4FCfg<T, NN>::OK) {                                                      \
+      using CF = S4FCfg<T, NN>;                                                             \
+      a.lpw = CF::W;                                                                        \
+      dim3 g((unsigned)((a.nlines + CF::W - 1) / CF::W)), b(CF::TPB);                       \
+      if (mode == M_MULT) hipLaunchKernelGGL((sfft4f_kernel<T, NN, M_MULT>), g, b, 0, st, a, tab.tw_sub); \
+      else hipLaunchKernelGGL((sfft4f_kernel<T, NN, M_DERIV>), g, b, 0, st, a, tab.tw_sub); \
+      return hipGetLastError();                                                             \
+    }                                                                                       \
+  }
+  AFFT_S4F(512)
+  AFFT_S4F(1024)
+  AFFT_S4F(2048)
+#undef AFFT_S4F
+  return hipErrorNotSupported;
+}
+
 // Launch one axis pass.  strided = false: contiguous lines (z layout).
 // Returns hipErrorNotSupported when the length does not fit one LDS tile.
 template <typename T>
@@ -281,6 +316,10 @@ hipError_t launch_fft(bool strided, int mode, int sign, FftArgs<T> a, const Axis
   if (tab.is_static) {
     a.tw = tab.tw_static;
     int per;
+    if (strided && (mode == M_MULT || mode == M_DERIV) && tab.tw_sub && four_step<T>(tab)) {
+      const hipError_t e = launch_s4f<T>(mode, a, tab, st);
+      if (e != hipErrorNotSupported) return e;
+    }
     if (strided && mode == M_C2C && tab.tw_sub && four_step<T>(tab)) {
       switch (n) {
         case 512: return launch_s4<T, 512>(sign, a, tab, st);

@@ -286,6 +286,64 @@ def run_wide(steps):
     return out
 
 
+def run_ops(steps):
+    """The spectral operators at 512^3 FP32 (accfft_grad_gpu with 3 outputs,
+    laplace, inverse laplace, biharmonic, divergence, the step4 Poisson
+    solve) and the differentiable loss 1/2 |div(grad(invlap u)) - g|^2 at
+    256^3 FP32 with its adjoint backward.  Bytes are this library's passes
+    (each operand read and written once per pass): grad 3 x 2R (one fused
+    per-axis derivative pass per component), the spectral operators 2R + 8C
+    (z r2c, y, x forward + symbol + inverse in one pass, y, z c2r),
+    divergence 8R (2R + two accumulating passes of 3R)."""
+    import torch
+    import accfft_amd as af
+    from accfft_amd import autograd as ag
+
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(20261206)
+    n = (512, 512, 512)
+    R = n[0] * n[1] * n[2] * 4
+    C = n[0] * n[1] * (n[2] // 2 + 1) * 8
+    u = torch.randn(n, generator=g, device=dev)
+    h = torch.empty((n[0], n[1], n[2] // 2 + 1), dtype=torch.complex64, device=dev)
+    o = [torch.empty_like(u) for _ in range(3)]
+    plan = af.Plan("r2c", n, u, h, af.F32)
+    out = {}
+    ops = [
+        ("grad_3_components", lambda: plan.grad(u, o[0], o[1], o[2]), 6 * R),
+        ("laplace", lambda: plan.laplace(u, o[0]), 2 * R + 8 * C),
+        ("inv_laplace", lambda: plan.inv_laplace(u, o[0]), 2 * R + 8 * C),
+        ("biharmonic", lambda: plan.biharmonic(u, o[0]), 2 * R + 8 * C),
+        ("poisson_solve", lambda: plan.poisson_solve(u, o[0], 1), 2 * R + 8 * C),
+    ]
+    div = torch.empty_like(u)
+    ops.append(("divergence", lambda: plan.divergence(o[0], o[1], o[2], div), 8 * R))
+    for name, fn, b in ops:
+        t = _timeit(fn, steps)
+        out[name] = {"ms": round(t, 4), "alg_GBps": round(b / (t * 1e-3) / 1e9, 1),
+                     "frac_of_hbm_peak": round(b / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        log(f"  {name:18s} {t:.3f} ms  {b / (t * 1e-3) / 1e9:7.0f} GB/s")
+    plan.destroy()
+    del u, h, o
+    torch.cuda.empty_cache()
+    # differentiation: forward + backward of the loss at 256^3 FP32
+    m = (256, 256, 256)
+    S = ag.context(m, torch.float32, dev)
+    v = torch.randn(m, generator=g, device=dev, requires_grad=True)
+    tgt = torch.randn(m, generator=g, device=dev)
+
+    def fwd_bwd():
+        v.grad = None
+        r = ag.divergence(*ag.grad(ag.inv_laplace(v, S), S), S) - tgt
+        loss = 0.5 * (r * r).sum()
+        loss.backward()
+    t = _timeit(fwd_bwd, steps)
+    out["loss_fwd_bwd_256"] = {"ms": round(t, 4)}
+    log(f"  loss fwd+bwd 256^3  {t:.3f} ms")
+    return out
+
+
 def cpu_baseline(n, budget_s=30.0):
     """FP64 oracle round trip on the host, all allowed threads."""
     import numpy as np
@@ -331,6 +389,8 @@ def main():
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision leg")
     ap.add_argument("--no-wide", action="store_true",
                     help="skip the slab-share and 1024^3 legs")
+    ap.add_argument("--no-ops", action="store_true",
+                    help="skip the operator and differentiation leg")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -370,6 +430,9 @@ def main():
     if not args.no_wide:
         log("larger shapes:")
         extra["wide"] = run_wide(max(5, args.steps // 2))
+    if not args.no_ops:
+        log("operators (512^3 FP32) and differentiation (256^3 FP32):")
+        extra["operators"] = run_ops(max(5, args.steps // 2))
 
     out = {
         "metric": "512^3 r2c+c2r ms & %HBM peak; split-BF16 vs FP32 speedup at rel-err<=1e-5",

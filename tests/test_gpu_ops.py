@@ -216,3 +216,45 @@ def test_axpy(dev):
     torch.cuda.synchronize()
     # the kernel may contract to one FMA: agree to a few ulp
     assert torch.allclose(y, ref, rtol=1e-15, atol=1e-14)
+
+
+@pytest.mark.parametrize("shape", [(512, 24, 40), (24, 1024, 34), (2048, 6, 10), (512, 8, 35)])
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_fused_two_level_operator_passes(dev, orc, shape, prec):
+    """Strided axes of 512 / 1024 / 2048 points: the fused two-level tiles
+    (forward FFT, symbol, inverse FFT in one pass) of FP32 plans for the
+    spectral operators (x pass), grad / divergence (per-axis derivatives,
+    complex view of column pairs; the odd z length takes the one-level
+    tile) and the slow variants (y symbol pass); FP64 keeps the one-level
+    tiles."""
+    import torch
+    import accfft_amd as af
+    print(f"fused operator passes {shape} {prec}", flush=True)
+    rdt, cdt, fp, tol = ((torch.float32, torch.complex64, af.F32, 1e-5) if prec == "f32"
+                         else (torch.float64, torch.complex128, af.F64, 1e-12))
+    a = rng(91).standard_normal(shape)
+    ad = torch.tensor(a, dtype=rdt, device=dev)
+    h = torch.empty((shape[0], shape[1], shape[2] // 2 + 1), dtype=cdt, device=dev)
+    plan = af.Plan("r2c", shape, ad, h, fp)
+    a64 = ad.cpu().numpy().astype(np.float64)
+    out = torch.empty_like(ad)
+    for name, ref in (("laplace", orc.laplace(a64)), ("inv_laplace", orc.inv_laplace(a64)),
+                      ("biharmonic", orc.biharmonic(a64))):
+        getattr(plan, name)(ad, out)
+        torch.cuda.synchronize()
+        assert rel_l2(out.cpu().numpy(), ref) < tol, name
+    plan.poisson_solve(ad, out, 1)
+    torch.cuda.synchronize()
+    assert rel_l2(out.cpu().numpy(), orc.poisson_solve(a64, 1)) < tol
+    for slow in (False, True):
+        g = [torch.empty_like(ad) for _ in range(3)]
+        plan.grad(ad, *g, slow=slow)
+        torch.cuda.synchronize()
+        refs = orc.grad(a64, 7, slow)
+        for d in range(3):
+            assert rel_l2(g[d].cpu().numpy(), refs[d]) < tol, ("grad", slow, d)
+        plan.divergence(g[0], g[1], g[2], out, slow=slow)
+        torch.cuda.synchronize()
+        gs = [x.cpu().numpy().astype(np.float64) for x in g]
+        assert rel_l2(out.cpu().numpy(), orc.divergence(*gs, slow)) < tol, ("div", slow)
+    plan.destroy()
