@@ -1010,20 +1010,50 @@ __global__ void __launch_bounds__((S4FCfg<T, N>::TPB))
       Z = col % a.Ch;
     }
     // one loop per symbol kind (the kind is uniform: the switch stays out
-    // of the loop and apply_sym folds to that kind's arithmetic)
+    // of the loop).  The grad and laplace-family symbols take a fast form:
+    // the wavenumbers of the two fixed axes once per thread, the varying
+    // one per element, 32-bit integers and T arithmetic (|k|^2 sums below
+    // 2^24 are exact in FP32); the Poisson symbols use apply_sym.
+    const int Nk = (int)(a.maxis == 0 ? a.N0 : a.N1), Nf = (int)(a.maxis == 0 ? a.N1 : a.N0);
+    const int N2i = (int)a.N2;
+    auto nyq0 = [](int x, int n) { int w = x > n / 2 ? x - n : x; return x == n / 2 ? 0 : w; };
+    auto plain = [](int x, int n) { return x > n / 2 ? x - n : x; };
+    const int wf0 = nyq0((int)u, Nf), wz0 = nyq0((int)Z, N2i);
+    const int wf1 = plain((int)u, Nf), wz1 = plain((int)Z, N2i);
+    const T sc = (T)a.scale;
     auto sym_loop = [&](auto kind) {
+      constexpr int KIND = decltype(kind)::value;
 #pragma unroll
       for (int i = 0; i < R1 * D1; ++i) {
         const int e = t + i * TPB;
         const int k2 = (e / W) % M, k1 = e / (M * W);
-        const long k = (long)(k1 + R1 * k2);
+        const int k = k1 + R1 * k2;
         const cpx<T> x = buf[e];
         if constexpr (MODE == M_DERIV) {
-          const T f = (T)(a.scale * (double)wave_sym(k, N));
+          const T f = sc * (T)wave_sym(k, N);
           buf[e] = cpx<T>{-f * x.y, f * x.x};
+        } else if constexpr (KIND <= SYM_GRAD_Z) {
+          // i * w_d * scale, w_d of the varying axis or a fixed one
+          const bool on_k = (KIND == SYM_GRAD_X && a.maxis == 0) || (KIND == SYM_GRAD_Y && a.maxis == 1);
+          const int w = on_k ? nyq0(k, Nk) : KIND == SYM_GRAD_Z ? wz0 : wf0;
+          const T f = sc * (T)w;
+          buf[e] = cpx<T>{-f * x.y, f * x.x};
+        } else if constexpr (KIND <= SYM_INV_LAPLACE) {
+          const int wk = nyq0(k, Nk);
+          const T wave = -(T)(wk * wk + wf0 * wf0 + wz0 * wz0);
+          T f;
+          if constexpr (KIND == SYM_LAPLACE) f = sc * wave;
+          else if constexpr (KIND == SYM_BIHARMONIC) f = (sc * wave) * wave;
+          else f = wave == (T)0 ? (T)0 : sc / wave;
+          buf[e] = cpx<T>{f * x.x, f * x.y};
+        } else if constexpr (KIND == SYM_INV_BIHARMONIC) {
+          const int wk = plain(k, Nk);
+          const T wave = (T)(wk * wk + wf1 * wf1 + wz1 * wz1);
+          const T f = wave == (T)0 ? sc : (sc / wave) / wave;
+          buf[e] = cpx<T>{f * x.x, f * x.y};
         } else {
-          const long X = a.maxis == 0 ? k : (long)u, Y = a.maxis == 0 ? (long)u : k;
-          buf[e] = apply_sym<T>(decltype(kind)::value, X, Y, (long)Z, a.N0, a.N1, a.N2, a.scale, x);
+          const long X = a.maxis == 0 ? (long)k : (long)u, Y = a.maxis == 0 ? (long)u : (long)k;
+          buf[e] = apply_sym<T>(KIND, X, Y, (long)Z, a.N0, a.N1, a.N2, a.scale, x);
         }
       }
     };
@@ -1060,16 +1090,21 @@ __global__ void __launch_bounds__((S4FCfg<T, N>::TPB))
     if (ok) {
       cpx<T>* p = dst + (int64_t)n2 * a.Sout;
       const int64_t step = (int64_t)M * a.Sout;
+      if (MODE == M_DERIV && a.accumulate) {  // uniform: one loop per case
 #pragma unroll
-      for (int n1 = 0; n1 < R1; ++n1) {
-        if (MODE == M_DERIV && a.accumulate) {
+        for (int n1 = 0; n1 < R1; ++n1) {
           const cpx<T> o = *p;
           *p = cpx<T>{o.x + w[n1].x, o.y + w[n1].y};
-        } else {
-          *p = w[n1];
+          p += step;
+          asm volatile("" : "+v"(p));
         }
-        p += step;
-        asm volatile("" : "+v"(p));
+      } else {
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1) {
+          *p = w[n1];
+          p += step;
+          asm volatile("" : "+v"(p));
+        }
       }
     }
   }
