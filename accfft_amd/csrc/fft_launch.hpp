@@ -71,19 +71,19 @@ hipError_t launch_z_n(int mode, int sign, FftArgs<T>& a, int64_t grid, size_t sh
   return hipGetLastError();
 }
 
-template <typename T, int N>
+template <typename T, int N, int LDSK = 64>
 hipError_t launch_s_n(int mode, int sign, FftArgs<T>& a, int64_t grid, size_t shm,
                       hipStream_t st) {
   dim3 g((unsigned)grid), b(256);
   switch (mode) {
     case M_C2C:
-      if (sign < 0) hipLaunchKernelGGL((sfft_kernel<T, N, M_C2C, -1>), g, b, shm, st, a);
-      else hipLaunchKernelGGL((sfft_kernel<T, N, M_C2C, +1>), g, b, shm, st, a);
+      if (sign < 0) hipLaunchKernelGGL((sfft_kernel<T, N, M_C2C, -1, 0, LDSK>), g, b, shm, st, a);
+      else hipLaunchKernelGGL((sfft_kernel<T, N, M_C2C, +1, 0, LDSK>), g, b, shm, st, a);
       break;
-    case M_R2C: hipLaunchKernelGGL((sfft_kernel<T, N, M_R2C, -1>), g, b, shm, st, a); break;
-    case M_C2R: hipLaunchKernelGGL((sfft_kernel<T, N, M_C2R, +1>), g, b, shm, st, a); break;
-    case M_DERIV: hipLaunchKernelGGL((sfft_kernel<T, N, M_DERIV, -1>), g, b, shm, st, a); break;
-    case M_MULT: hipLaunchKernelGGL((sfft_kernel<T, N, M_MULT, -1>), g, b, shm, st, a); break;
+    case M_R2C: hipLaunchKernelGGL((sfft_kernel<T, N, M_R2C, -1, 0, LDSK>), g, b, shm, st, a); break;
+    case M_C2R: hipLaunchKernelGGL((sfft_kernel<T, N, M_C2R, +1, 0, LDSK>), g, b, shm, st, a); break;
+    case M_DERIV: hipLaunchKernelGGL((sfft_kernel<T, N, M_DERIV, -1, 0, LDSK>), g, b, shm, st, a); break;
+    case M_MULT: hipLaunchKernelGGL((sfft_kernel<T, N, M_MULT, -1, 0, LDSK>), g, b, shm, st, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -111,13 +111,16 @@ inline void dyn_lds_attrs() {
 
 // Strided C2C tiles of half the LDS (32 KiB: twice the workgroups per CU,
 // half-width row segments) for the lengths whose bit log2(n) is set in
-// ACCFFT_AMD_SNARROW (read once; default none).
+// ACCFFT_AMD_SNARROW (read once).  Default: 128 and 256, whose passes are
+// too short to fill the chip twice over with 64 KiB tiles (FP32 256^3:
+// strided passes 38 -> 28 us, round trip 0.206 -> 0.172 ms; 128^3 0.072 ->
+// 0.064 ms; FP64 tiles are the same either way).
 inline bool narrow_tiles(int n) {
   static const unsigned mask = [] {
     const char* e = getenv("ACCFFT_AMD_SNARROW");
-    return e ? (unsigned)strtoul(e, nullptr, 0) : 0u;
+    return e ? (unsigned)strtoul(e, nullptr, 0) : (1u << 7) | (1u << 8);
   }();
-  if (n != 512 && n != 1024 && n != 2048) return false;
+  if (n != 128 && n != 256 && n != 512 && n != 1024 && n != 2048) return false;
   int l = 0;
   while ((1 << l) < n) ++l;
   return (mask >> l) & 1u;
@@ -346,7 +349,7 @@ hipError_t launch_fft(bool strided, int mode, int sign, FftArgs<T> a, const Axis
       AFFT_WIDE(4096)
 #undef AFFT_WIDE
     }
-    if (strided && mode == M_C2C && narrow_tiles(n)) {
+    if (strided && mode == M_C2C && n >= 512 && narrow_tiles(n)) {
       const int w = n == 512 ? SCfg<T, 512, 0, 32>::W
                   : n == 1024 ? SCfg<T, 1024, 0, 32>::W : SCfg<T, 2048, 0, 32>::W;
       a.lpw = w;
@@ -365,6 +368,12 @@ hipError_t launch_fft(bool strided, int mode, int sign, FftArgs<T> a, const Axis
     }
 #define AFFT_CASE(NN)                                                              \
   case NN:                                                                         \
+    if constexpr (NN == 128 || NN == 256)                                          \
+      if (strided && narrow_tiles(NN)) {                                           \
+        per = SCfg<T, NN, 0, 32>::W;                                               \
+        a.lpw = per;                                                               \
+        return launch_s_n<T, NN, 32>(mode, sign, a, (units + per - 1) / per, 0, st); \
+      }                                                                            \
     per = strided ? SCfg<T, NN>::W : ZCfg<NN>::LPW;                                \
     a.lpw = per;                                                                   \
     return strided ? launch_s_n<T, NN>(mode, sign, a, (units + per - 1) / per, 0, st) \
