@@ -264,19 +264,28 @@ __host__ __device__ constexpr int sched_minnb() {  // fewest butterflies in any 
   return m;
 }
 
-// Accessor over one line in LDS with compile-time element stride ES.
-template <typename T, int ES>
+// LDS index of element e of a tile whose lines are padded by one element
+// after every 16 (lines of 16k elements; line p starts at 17k p).  A
+// Stockham stage writes with a lane stride of R elements (R = 8, 16): padded,
+// the lanes of a 32-lane (8-byte elements) or 16-lane (16-byte elements) LDS
+// access fall on distinct banks instead of 2 to 16 of them.
+__device__ __forceinline__ int lds_pad16(int e) { return e + (e >> 4); }
+
+// Accessor over one line in LDS with compile-time element stride ES; PAD:
+// the line is stored padded (lds_pad16, ES = 1 only).
+template <typename T, int ES, int PAD = 0>
 struct LineAcc {
+  static_assert(!PAD || ES == 1, "padded lines are contiguous");
   cpx<T>* p;
-  __device__ __forceinline__ cpx<T> ld(int n) const { return p[n * ES]; }
-  __device__ __forceinline__ void st(int n, cpx<T> v) const { p[n * ES] = v; }
+  __device__ __forceinline__ cpx<T> ld(int n) const { return p[PAD ? lds_pad16(n) : n * ES]; }
+  __device__ __forceinline__ void st(int n, cpx<T> v) const { p[PAD ? lds_pad16(n) : n * ES] = v; }
 };
 
 // One in-place stage.  Thread q of NT threads on the line performs
 // butterflies j = q + NT*i.  Twiddles for stage s live at tw[TWOFF +
 // (r-1)*NS + k] = exp(-2 pi i r k / (NS R)); conjugated for SIGN = +1.
-template <typename T, int N, int R, int NS, int NT, int SIGN, int TWOFF, int ES>
-__device__ __forceinline__ void sstage(const LineAcc<T, ES>& acc, int q,
+template <typename T, int N, int R, int NS, int NT, int SIGN, int TWOFF, int ES, int PAD = 0>
+__device__ __forceinline__ void sstage(const LineAcc<T, ES, PAD>& acc, int q,
                                        const cpx<T>* __restrict__ tw) {
   constexpr int NB = N / R;
   constexpr int PER = NB / NT;

@@ -61,10 +61,10 @@ __device__ __forceinline__ int64_t row_off(int r, int64_t S, int shift, int64_t 
 }
 
 // -------------------------------------------------------------- configs
-template <int N, int SPLIT = 0>
+template <int N, int SPLIT = 0, int ZT = 256>
 struct ZCfg {
   static constexpr int NT = sched_minnb<N, SPLIT>();
-  static constexpr int TPB = NT > 256 ? NT : 256;
+  static constexpr int TPB = NT > ZT ? NT : ZT;
   static constexpr int LPW = TPB / NT;
 };
 
@@ -153,13 +153,13 @@ __device__ __forceinline__ void stage_twiddles(cpx<T>* dst, const cpx<T>* __rest
 
 // -------------------------------------------------------------- engines
 // N > 0: static engine; N == 0: dynamic engine.
-template <typename T, int N, int NT, int SIGN, int SPLIT = 0>
+template <typename T, int N, int NT, int SIGN, int SPLIT = 0, int LINES = 256 / NT, int PAD = 0>
 __device__ __forceinline__ cpx<T>* zrun(cpx<T>* A, cpx<T>* B, const FftArgs<T>& a,
                                         const cpx<T>* twl, int lpw) {
   if constexpr (N > 0) {
     const int line = threadIdx.x / NT, q = threadIdx.x % NT;
-    LineAcc<T, 1> acc{A + line * N};
-    static_fft<T, N, NT, SIGN, 1, SPLIT, 256 / NT>(acc, q, twl, A, N);
+    LineAcc<T, 1, PAD> acc{A + line * (PAD ? N + N / 16 : N)};
+    static_fft<T, N, NT, SIGN, 1, SPLIT, LINES>(acc, q, twl, A, N);
     return A;
   } else {
     LayZ lay{a.sc.n};
@@ -184,11 +184,13 @@ __device__ __forceinline__ cpx<T>* srun(cpx<T>* A, cpx<T>* B, const FftArgs<T>& 
 // ====================================================================
 // Contiguous lines (z)
 // ====================================================================
-template <typename T, int N, int MODE, int SIGN, int SPLIT = 0>
-__global__ void __launch_bounds__(256) zfft_kernel(FftArgs<T> a) {
+// ZT: threads per block (lines per block = ZT / threads per line).
+template <typename T, int N, int MODE, int SIGN, int SPLIT = 0, int ZT = 256>
+__global__ void __launch_bounds__(ZT > 256 ? ZT : 256) zfft_kernel(FftArgs<T> a) {
   constexpr bool STATIC = N > 0;
   constexpr int NS_ = STATIC ? N : 16;
-  using ZC = ZCfg<NS_, SPLIT>;
+  static_assert(ZT == 256 || SPLIT == 0, "matrix-core stages assume 256-thread blocks");
+  using ZC = ZCfg<NS_, SPLIT, ZT>;
   constexpr int NT = ZC::NT;
   constexpr int TPB = STATIC ? ZC::TPB : 256;
   constexpr int TWN = STATIC ? sched_twn<NS_, SPLIT>() : 0;
@@ -197,13 +199,16 @@ __global__ void __launch_bounds__(256) zfft_kernel(FftArgs<T> a) {
   const int nh = n / 2 + 1;
   const int lpw = STATIC ? ZC::LPW : a.lpw;  // complex lines per block
   const int t = threadIdx.x;
+  // static tiles are stored padded (lds_pad16): conflict-free stage writes
+  constexpr int ZPAD = (STATIC && SPLIT == 0) ? 1 : 0;
+  auto P = [](int e) { return ZPAD ? lds_pad16(e) : e; };
 
   cpx<T>* bufA;
   cpx<T>* bufB;
   const cpx<T>* twl;
   if constexpr (STATIC) {
     constexpr int LPW = ZC::LPW;
-    __shared__ cpx<T> s_buf[LPW * NS_];
+    __shared__ cpx<T> s_buf[LPW * NS_ + (ZPAD ? LPW * NS_ / 16 : 0)];
     __shared__ cpx<T> s_tw[TWN > 0 ? TWN : 1];
     bufA = s_buf;
     bufB = s_buf;
@@ -260,8 +265,8 @@ __global__ void __launch_bounds__(256) zfft_kernel(FftArgs<T> a) {
             x = cscale(x, (T)0.5);
             y = cscale(y, (T)0.5);
           }
-          bufA[p * NS_ + k] = cpx<T>{x.x - y.y, x.y + y.x};
-          if (k != 0 && 2 * k != NS_) bufA[p * NS_ + NS_ - k] = cpx<T>{x.x + y.y, y.x - x.y};
+          bufA[P(p * NS_ + k)] = cpx<T>{x.x - y.y, x.y + y.x};
+          if (k != 0 && 2 * k != NS_) bufA[P(p * NS_ + NS_ - k)] = cpx<T>{x.x + y.y, y.x - x.y};
         }
       }
     } else {
@@ -281,7 +286,7 @@ __global__ void __launch_bounds__(256) zfft_kernel(FftArgs<T> a) {
     }
     stage_twiddles<T, TWN, TPB>(const_cast<cpx<T>*>(twl), a.tw, t);
 #pragma unroll
-    for (int j = 0; j < EP; ++j) bufA[t + j * TPB] = v[j];
+    for (int j = 0; j < EP; ++j) bufA[P(t + j * TPB)] = v[j];
     }
   } else if constexpr (MODE == M_C2C || MODE == M_MULT) {
     for (int e = t; e < lpw * n; e += TPB) {
@@ -323,13 +328,13 @@ __global__ void __launch_bounds__(256) zfft_kernel(FftArgs<T> a) {
   // ---------------- transform
   cpx<T>* res;
   if constexpr (MODE == M_C2C) {
-    res = zrun<T, N, NT, SIGN, SPLIT>(bufA, bufB, a, twl, lpw);
+    res = zrun<T, N, NT, SIGN, SPLIT, ZC::LPW, ZPAD>(bufA, bufB, a, twl, lpw);
   } else if constexpr (MODE == M_R2C) {
-    res = zrun<T, N, NT, -1, SPLIT>(bufA, bufB, a, twl, lpw);
+    res = zrun<T, N, NT, -1, SPLIT, ZC::LPW, ZPAD>(bufA, bufB, a, twl, lpw);
   } else if constexpr (MODE == M_C2R) {
-    res = zrun<T, N, NT, +1, SPLIT>(bufA, bufB, a, twl, lpw);
+    res = zrun<T, N, NT, +1, SPLIT, ZC::LPW, ZPAD>(bufA, bufB, a, twl, lpw);
   } else {  // M_DERIV and M_MULT (z has no MULT use; treat as DERIV-style symbol)
-    res = zrun<T, N, NT, -1, SPLIT>(bufA, bufB, a, twl, lpw);
+    res = zrun<T, N, NT, -1, SPLIT, ZC::LPW, ZPAD>(bufA, bufB, a, twl, lpw);
     const int epm = STATIC ? ZC::LPW * NS_ / TPB : (lpw * n + TPB - 1) / TPB;
 #pragma unroll
     for (int j = 0; j < epm; ++j) {
@@ -337,12 +342,12 @@ __global__ void __launch_bounds__(256) zfft_kernel(FftArgs<T> a) {
       if (!STATIC && e >= lpw * n) break;
       const int i = e % n;
       const T f = (T)(a.scale * (double)wave_sym(i, n));
-      const cpx<T> v = res[e];
-      res[e] = {-f * v.y, f * v.x};
+      const cpx<T> v = res[P(e)];
+      res[P(e)] = {-f * v.y, f * v.x};
     }
     __syncthreads();
     cpx<T>* other = (res == bufA) ? bufB : bufA;
-    res = zrun<T, N, NT, +1, SPLIT>(res, other, a, twl, lpw);
+    res = zrun<T, N, NT, +1, SPLIT, ZC::LPW, ZPAD>(res, other, a, twl, lpw);
   }
 
   // ---------------- store
@@ -358,7 +363,7 @@ __global__ void __launch_bounds__(256) zfft_kernel(FftArgs<T> a) {
       if (!STATIC && e >= lpw * n) break;
       const int p = e / n, i = e - p * n;
       const int64_t g = l0 + p;
-      if (g < nl) stc<T>(a.out, g * a.odist + i, res[e]);
+      if (g < nl) stc<T>(a.out, g * a.odist + i, res[P(e)]);
     }
   } else if constexpr (MODE == M_R2C) {
 #pragma unroll
@@ -369,8 +374,8 @@ __global__ void __launch_bounds__(256) zfft_kernel(FftArgs<T> a) {
       const int p = rl >> 1;
       const int64_t g = 2 * l0 + rl;
       if (g >= nl) continue;
-      const cpx<T> z1 = res[p * n + k];
-      const cpx<T> z2 = res[p * n + (k == 0 ? 0 : n - k)];
+      const cpx<T> z1 = res[P(p * n + k)];
+      const cpx<T> z2 = res[P(p * n + (k == 0 ? 0 : n - k))];
       const T h = (T)0.5;
       cpx<T> o;
       if ((rl & 1) == 0) o = {h * (z1.x + z2.x), h * (z1.y - z2.y)};
@@ -385,7 +390,7 @@ __global__ void __launch_bounds__(256) zfft_kernel(FftArgs<T> a) {
       if (!STATIC && e >= lpw * n) break;
       const int p = e / n, i = e - p * n;
       const int64_t ra = 2 * (l0 + p), rb = ra + 1;
-      const cpx<T> z = res[e];
+      const cpx<T> z = res[P(e)];
       if constexpr (MODE == M_DERIV) {
         if (a.accumulate) {
           if (ra < nl) str<T>(a.out, ra * a.odist + i, ldr<T>(a.out, ra * a.odist + i) + z.x);

@@ -54,18 +54,18 @@ inline int dyn_cols(int n) {
 inline size_t dyn_z_lds(int n, int lpw, size_t csz) { return 2 * (size_t)lpw * n * csz; }
 inline size_t dyn_s_lds(int n, int w, size_t csz) { return 2 * (size_t)w * n * csz; }
 
-template <typename T, int N>
+template <typename T, int N, int ZT = 256>
 hipError_t launch_z_n(int mode, int sign, FftArgs<T>& a, int64_t grid, size_t shm,
                       hipStream_t st) {
-  dim3 g((unsigned)grid), b(256);
+  dim3 g((unsigned)grid), b(N > 0 ? ZCfg<(N > 0 ? N : 16), 0, ZT>::TPB : 256);
   switch (mode) {
     case M_C2C:
-      if (sign < 0) hipLaunchKernelGGL((zfft_kernel<T, N, M_C2C, -1>), g, b, shm, st, a);
-      else hipLaunchKernelGGL((zfft_kernel<T, N, M_C2C, +1>), g, b, shm, st, a);
+      if (sign < 0) hipLaunchKernelGGL((zfft_kernel<T, N, M_C2C, -1, 0, ZT>), g, b, shm, st, a);
+      else hipLaunchKernelGGL((zfft_kernel<T, N, M_C2C, +1, 0, ZT>), g, b, shm, st, a);
       break;
-    case M_R2C: hipLaunchKernelGGL((zfft_kernel<T, N, M_R2C, -1>), g, b, shm, st, a); break;
-    case M_C2R: hipLaunchKernelGGL((zfft_kernel<T, N, M_C2R, +1>), g, b, shm, st, a); break;
-    case M_DERIV: hipLaunchKernelGGL((zfft_kernel<T, N, M_DERIV, -1>), g, b, shm, st, a); break;
+    case M_R2C: hipLaunchKernelGGL((zfft_kernel<T, N, M_R2C, -1, 0, ZT>), g, b, shm, st, a); break;
+    case M_C2R: hipLaunchKernelGGL((zfft_kernel<T, N, M_C2R, +1, 0, ZT>), g, b, shm, st, a); break;
+    case M_DERIV: hipLaunchKernelGGL((zfft_kernel<T, N, M_DERIV, -1, 0, ZT>), g, b, shm, st, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -124,6 +124,18 @@ inline bool narrow_tiles(int n) {
   int l = 0;
   while ((1 << l) < n) ++l;
   return (mask >> l) & 1u;
+}
+
+// Threads per block of the contiguous-line passes of length 128 and 256
+// (128, 256 or 512; ACCFFT_AMD_ZT, read once; default 256).
+template <typename T>
+inline int z_threads(int n) {
+  static const int zt = [] {
+    const char* e = getenv("ACCFFT_AMD_ZT");
+    return e ? atoi(e) : 256;
+  }();
+  (void)n;
+  return zt;
 }
 
 // Strided C2C tiles of 128 KiB of LDS (one workgroup per CU, twice the row
@@ -368,12 +380,24 @@ hipError_t launch_fft(bool strided, int mode, int sign, FftArgs<T> a, const Axis
     }
 #define AFFT_CASE(NN)                                                              \
   case NN:                                                                         \
-    if constexpr (NN == 128 || NN == 256)                                          \
+    if constexpr (NN == 128 || NN == 256) {                                        \
       if (strided && narrow_tiles(NN)) {                                           \
         per = SCfg<T, NN, 0, 32>::W;                                               \
         a.lpw = per;                                                               \
         return launch_s_n<T, NN, 32>(mode, sign, a, (units + per - 1) / per, 0, st); \
       }                                                                            \
+      const int zt = z_threads<T>(NN);                                             \
+      if (!strided && zt == 128) {                                                 \
+        per = ZCfg<NN, 0, 128>::LPW;                                               \
+        a.lpw = per;                                                               \
+        return launch_z_n<T, NN, 128>(mode, sign, a, (units + per - 1) / per, 0, st); \
+      }                                                                            \
+      if (!strided && zt == 512) {                                                 \
+        per = ZCfg<NN, 0, 512>::LPW;                                               \
+        a.lpw = per;                                                               \
+        return launch_z_n<T, NN, 512>(mode, sign, a, (units + per - 1) / per, 0, st); \
+      }                                                                            \
+    }                                                                              \
     per = strided ? SCfg<T, NN>::W : ZCfg<NN>::LPW;                                \
     a.lpw = per;                                                                   \
     return strided ? launch_s_n<T, NN>(mode, sign, a, (units + per - 1) / per, 0, st) \

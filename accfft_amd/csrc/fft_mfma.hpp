@@ -241,11 +241,12 @@ __global__ void __launch_bounds__(64) dft16_selftest_kernel(const cpx<float>* x,
 // acc/q: this thread's line and index within it (vector stages);
 // tile/LS: the whole tile (matrix-core stages).  Caller must
 // __syncthreads() before (data and twiddles staged).
-template <typename T, int N, int NT, int SIGN, int ES, int SPLIT, int LINES>
-__device__ __forceinline__ void static_fft(const LineAcc<T, ES>& acc, int q,
+template <typename T, int N, int NT, int SIGN, int ES, int SPLIT, int LINES, int PAD = 0>
+__device__ __forceinline__ void static_fft(const LineAcc<T, ES, PAD>& acc, int q,
                                            const cpx<T>* __restrict__ tw, cpx<T>* tile, int LS) {
   using S = SchedOf<N, SPLIT>;
   constexpr int R0 = S::r(0), R1 = S::r(1), R2 = S::r(2);
+  static_assert(!PAD || !split_sched(SPLIT), "matrix-core stages read the unpadded tile");
   if constexpr (split_sched(SPLIT) && R0 == 16)
     mstage<N, 1, SIGN, 0, ES, LINES, SPLIT>(tile, LS, tw);
   else
