@@ -384,6 +384,31 @@ __global__ void __launch_bounds__(ZT > 256 ? ZT : 256) zfft_kernel(FftArgs<T> a)
       stc<T>(a.out, g * a.odist + k, o);
     }
   } else {  // real outputs
+    if constexpr (MODE == M_DERIV && STATIC) {
+      if (a.accumulate) {
+        // all of the thread's accumulator loads in flight together, then
+        // the stores (a load-add-store per element would wait on each load)
+        cpx<T> o[EPS];
+#pragma unroll
+        for (int j = 0; j < EPS; ++j) {
+          const int e = t + j * TPB;
+          const int p = e / n, i = e - p * n;
+          const int64_t ra = 2 * (l0 + p), rb = ra + 1;
+          o[j].x = ra < nl ? ldr<T>(a.out, ra * a.odist + i) : (T)0;
+          o[j].y = rb < nl ? ldr<T>(a.out, rb * a.odist + i) : (T)0;
+        }
+#pragma unroll
+        for (int j = 0; j < EPS; ++j) {
+          const int e = t + j * TPB;
+          const int p = e / n, i = e - p * n;
+          const int64_t ra = 2 * (l0 + p), rb = ra + 1;
+          const cpx<T> z = res[P(e)];
+          if (ra < nl) str<T>(a.out, ra * a.odist + i, o[j].x + z.x);
+          if (rb < nl) str<T>(a.out, rb * a.odist + i, o[j].y + z.y);
+        }
+        return;
+      }
+    }
 #pragma unroll
     for (int j = 0; j < ep; ++j) {
       const int e = t + j * TPB;
@@ -391,7 +416,7 @@ __global__ void __launch_bounds__(ZT > 256 ? ZT : 256) zfft_kernel(FftArgs<T> a)
       const int p = e / n, i = e - p * n;
       const int64_t ra = 2 * (l0 + p), rb = ra + 1;
       const cpx<T> z = res[P(e)];
-      if constexpr (MODE == M_DERIV) {
+      if constexpr (MODE == M_DERIV && !STATIC) {
         if (a.accumulate) {
           if (ra < nl) str<T>(a.out, ra * a.odist + i, ldr<T>(a.out, ra * a.odist + i) + z.x);
           if (rb < nl) str<T>(a.out, rb * a.odist + i, ldr<T>(a.out, rb * a.odist + i) + z.y);
@@ -561,11 +586,31 @@ __global__ void __launch_bounds__(256) sfft_kernel(FftArgs<T> a) {
       if (vb) stc<T>(a.out, ob + (int64_t)k * a.Sout, cpx<T>{h * (z1.y + z2.y), h * (z2.x - z1.x)});
     }
   } else {
+    if constexpr (MODE == M_DERIV && STATIC) {
+      if (a.accumulate) {
+        // the accumulator loads all in flight, then the stores
+        cpx<T> o[RP];
+#pragma unroll
+        for (int j = 0; j < RP; ++j) {
+          const int64_t off = (int64_t)(r0 + j * rstep) * a.Sout;
+          o[j].x = va ? ldr<T>(a.out, oa + off) : (T)0;
+          o[j].y = vb ? ldr<T>(a.out, ob + off) : (T)0;
+        }
+#pragma unroll
+        for (int j = 0; j < RP; ++j) {
+          const int r = r0 + j * rstep;
+          const cpx<T> z = res[r * W + w];
+          if (va) str<T>(a.out, oa + (int64_t)r * a.Sout, o[j].x + z.x);
+          if (vb) str<T>(a.out, ob + (int64_t)r * a.Sout, o[j].y + z.y);
+        }
+        return;
+      }
+    }
 #pragma unroll
     for (int j = 0; j < rp; ++j) {
       const int r = r0 + j * rstep;
       const cpx<T> z = res[r * W + w];
-      if constexpr (MODE == M_DERIV) {
+      if constexpr (MODE == M_DERIV && !STATIC) {
         if (a.accumulate) {
           if (va) str<T>(a.out, oa + (int64_t)r * a.Sout, ldr<T>(a.out, oa + (int64_t)r * a.Sout) + z.x);
           if (vb) str<T>(a.out, ob + (int64_t)r * a.Sout, ldr<T>(a.out, ob + (int64_t)r * a.Sout) + z.y);
@@ -1096,10 +1141,18 @@ __global__ void __launch_bounds__((S4FCfg<T, N>::TPB))
       cpx<T>* p = dst + (int64_t)n2 * a.Sout;
       const int64_t step = (int64_t)M * a.Sout;
       if (MODE == M_DERIV && a.accumulate) {  // uniform: one loop per case
+        // the accumulator loads all in flight, then the stores
+        cpx<T> o[R1];
+        const cpx<T>* pl = p;
 #pragma unroll
         for (int n1 = 0; n1 < R1; ++n1) {
-          const cpx<T> o = *p;
-          *p = cpx<T>{o.x + w[n1].x, o.y + w[n1].y};
+          o[n1] = *pl;
+          pl += step;
+          asm volatile("" : "+v"(pl));
+        }
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1) {
+          *p = cpx<T>{o[n1].x + w[n1].x, o[n1].y + w[n1].y};
           p += step;
           asm volatile("" : "+v"(p));
         }
