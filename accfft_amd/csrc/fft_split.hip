@@ -31,11 +31,12 @@ static hipError_t launch_split_n(bool strided, int mode, int sign, FftArgs<float
 }
 
 // Two-level strided tiles whose first radix-16 level runs on the matrix
-// cores (one-shot configurations that keep lanes l and l^32 in one column).
+// cores (configurations that keep lanes l and l^32 in one column; one-shot
+// or persistent).
 template <int N, int V>
 constexpr bool split_s4_ok() {
   using CF = S4Cfg<float, N, V>;
-  return CF::OK && CF::PB == 0 && CF::R1 == 16 && CF::TWF == 0 && 32 % CF::W == 0 && CF::TPB % 64 == 0 &&
+  return CF::OK && CF::R1 == 16 && CF::TWF == 0 && 32 % CF::W == 0 && CF::TPB % 64 == 0 &&
          (CF::KC == 8 || CF::KC == 16);
 }
 

@@ -127,3 +127,38 @@ def test_split_two_level_c2c(dev, orc, split, shape, direction):
     print(f"  rel-L2 {err:.3e}")
     assert err < BOUND3D[split]
     plan.destroy()
+
+
+@pytest.mark.parametrize("split", list(SPLITS))
+@pytest.mark.parametrize("shape", [(512, 512, 30), (1024, 6, 20), (2048, 4, 16)])
+def test_split_every_configuration(dev, orc, split, shape):
+    """Every tile configuration the MEASURE tuning may keep for a split
+    plan's strided axes (one-level, the two-level variants, persistent
+    tiles included) against the FP64 oracle, forward and round trip."""
+    import torch
+    import accfft_amd as af
+    a = torch.tensor(rng(83).standard_normal(shape), dtype=torch.float32, device=dev)
+    h = torch.empty((shape[0], shape[1], shape[2] // 2 + 1), dtype=torch.complex64, device=dev)
+    back = torch.empty_like(a)
+    plan = af.Plan("r2c", shape, a, h, af.F32, af.ACCFFT_ESTIMATE | SPLITS[split][1])
+    ref = orc.r2c(a.cpu().numpy().astype(np.float64))
+    for axis in (0, 1):
+        if shape[axis] not in (512, 1024, 2048):
+            continue
+        for c in (-2, 0, 1, 2, 3):
+            try:
+                plan.set_tuning(axis, c)
+            except Exception:
+                continue
+            print(f"split {split} {shape} axis {axis} choice {c}", flush=True)
+            plan.execute_r2c(a, h)
+            torch.cuda.synchronize()
+            err = rel_l2(h.cpu().numpy(), ref)
+            assert err < BOUND3D[split], (axis, c, err)
+            plan.execute_c2r(h, back)
+            torch.cuda.synchronize()
+            e = (torch.linalg.vector_norm(back / float(np.prod(shape)) - a) /
+                 torch.linalg.vector_norm(a)).item()
+            assert e < 2 * BOUND3D[split], (axis, c, e)
+        plan.set_tuning(axis, -1)
+    plan.destroy()
