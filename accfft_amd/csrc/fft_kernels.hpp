@@ -967,33 +967,40 @@ __global__ void __launch_bounds__((S4Cfg<T, N, V>::TPB), (S4Cfg<T, N, V>::WPE))
 //  sum_k2 X[k1 + R1 k2] e^{2 pi i k2 n2 / M}).  One read and one write of
 // the operand, like the one-level fused tile, but without its LDS
 // round trips of the first and last levels.
+// KC: values of k1 per LDS chunk in the M_DERIV kernel (the M-point levels
+// and the symbol run chunk by chunk; each thread keeps its k1 values in
+// registers between the levels); WPE: waves per SIMD its register
+// allocation must allow.
 template <typename T, int N>
 struct S4FCfg {
   static constexpr bool OK = false;
-  static constexpr int W = 1, TPB = 256;
+  static constexpr int W = 1, TPB = 256, KC = 16, WPE = 1;
 };
-#define AFFT_S4FCFG(TT, NN, W_, TPB_)                   \
+#define AFFT_S4FCFG(TT, NN, W_, TPB_, KC_, WPE_)        \
   template <>                                           \
   struct S4FCfg<TT, NN> {                               \
     static constexpr bool OK = true;                    \
-    static constexpr int W = W_, TPB = TPB_;            \
+    static constexpr int W = W_, TPB = TPB_, KC = KC_, WPE = WPE_; \
   };
-AFFT_S4FCFG(float, 512, 16, 512)
-AFFT_S4FCFG(float, 1024, 8, 512)
-AFFT_S4FCFG(float, 2048, 4, 512)
+AFFT_S4FCFG(float, 512, 16, 512, 8, 6)
+AFFT_S4FCFG(float, 1024, 8, 512, 8, 6)
+AFFT_S4FCFG(float, 2048, 4, 512, 16, 1)
 #undef AFFT_S4FCFG
 
 template <typename T, int N, int MODE>
-__global__ void __launch_bounds__((S4FCfg<T, N>::TPB))
+__global__ void __launch_bounds__((S4FCfg<T, N>::TPB), (MODE == M_DERIV ? S4FCfg<T, N>::WPE : 1))
     sfft4f_kernel(FftArgs<T> a, const cpx<T>* __restrict__ twsub) {
   using CF = S4FCfg<T, N>;
-  constexpr int R1 = 16, W = CF::W, TPB = CF::TPB, M = N / R1;
-  constexpr int D1 = W * M / TPB, RS1 = TPB / W, NT = TPB / (R1 * W);
+  // M_MULT keeps the whole tile in LDS (chunked, its symbol kinds need more
+  // registers than two workgroups per CU allow)
+  constexpr int R1 = 16, W = CF::W, TPB = CF::TPB, M = N / R1, KC = MODE == M_DERIV ? CF::KC : R1;
+  constexpr int D1 = W * M / TPB, RS1 = TPB / W, NT = TPB / (KC * W);
+  static_assert(R1 % KC == 0, "k1 chunks must split evenly");
   constexpr int TWM = sched_twn<M, 0>();
   static_assert(MODE == M_MULT || MODE == M_DERIV, "fused modes only");
   static_assert(D1 >= 1 && D1 * TPB == W * M, "tile must split evenly");
-  static_assert(NT >= 1 && NT * R1 * W == TPB, "lines must split evenly");
-  __shared__ cpx<T> buf[R1 * M * W];
+  static_assert(NT >= 1 && NT * KC * W == TPB, "lines must split evenly");
+  __shared__ cpx<T> buf[KC * M * W];
   __shared__ cpx<T> tw1[N];
   __shared__ cpx<T> tw2[TWM > 0 ? TWM : 1];
   const int t = threadIdx.x;
@@ -1036,19 +1043,27 @@ __global__ void __launch_bounds__((S4FCfg<T, N>::TPB))
     if (t + j * TPB < TWM) tw2[t + j * TPB] = tv2[j];
   __syncthreads();
 
-  // forward first level, then the whole tile to LDS
+  // forward first level in registers
 #pragma unroll
   for (int d = 0; d < D1; ++d) {
     dft<T, R1, -1>(v[d]);
     const int n2 = r0 + d * RS1;
 #pragma unroll
     for (int k1 = 1; k1 < R1; ++k1) v[d][k1] = cmul(v[d][k1], tw1[n2 * k1]);
-#pragma unroll
-    for (int k1 = 0; k1 < R1; ++k1) buf[(k1 * M + n2) * W + c] = v[d][k1];
   }
-  __syncthreads();
-  const int q = r0 % NT, kl = r0 / NT;  // line (k1 = kl, column c), thread q of NT
+  const int q = r0 % NT, kl = r0 / NT;  // line (k1 = kc0 + kl, column c), thread q of NT
   LineAcc<T, W> acc{buf + kl * M * W + c};
+  // per chunk of KC values of k1: to LDS, forward M-point level, symbol,
+  // inverse M-point level, back to the registers they came from (kc0 a
+  // compile-time constant, so that v stays in registers)
+  static_assert(KC == R1 || KC * 2 == R1, "one or two chunks");
+  auto chunk = [&](auto kc0c) {
+  constexpr int kc0 = decltype(kc0c)::value;
+#pragma unroll
+  for (int d = 0; d < D1; ++d)
+#pragma unroll
+    for (int kk = 0; kk < KC; ++kk) buf[(kk * M + r0 + d * RS1) * W + c] = v[d][kc0 + kk];
+  __syncthreads();
   static_fft<T, M, NT, -1, W, 0, 1>(acc, q, tw2, buf, 0);
 
   // symbol on X[k1 + R1 k2] at (k1, k2); thread t's elements all lie in
@@ -1074,9 +1089,9 @@ __global__ void __launch_bounds__((S4FCfg<T, N>::TPB))
     auto sym_loop = [&](auto kind) {
       constexpr int KIND = decltype(kind)::value;
 #pragma unroll
-      for (int i = 0; i < R1 * D1; ++i) {
+      for (int i = 0; i < KC * D1; ++i) {
         const int e = t + i * TPB;
-        const int k2 = (e / W) % M, k1 = e / (M * W);
+        const int k2 = (e / W) % M, k1 = kc0 + e / (M * W);
         const int k = k1 + R1 * k2;
         const cpx<T> x = buf[e];
         if constexpr (MODE == M_DERIV) {
@@ -1125,6 +1140,14 @@ __global__ void __launch_bounds__((S4FCfg<T, N>::TPB))
   }
   __syncthreads();
   static_fft<T, M, NT, +1, W, 0, 1>(acc, q, tw2, buf, 0);
+#pragma unroll
+  for (int d = 0; d < D1; ++d)
+#pragma unroll
+    for (int kk = 0; kk < KC; ++kk) v[d][kc0 + kk] = buf[(kk * M + r0 + d * RS1) * W + c];
+  __syncthreads();
+  };
+  chunk(std::integral_constant<int, 0>{});
+  if constexpr (KC < R1) chunk(std::integral_constant<int, KC>{});
 
   // inverse first level: twiddles, R1-point inverse DFTs, store
   cpx<T>* dst = reinterpret_cast<cpx<T>*>(a.out) + ob;
@@ -1133,7 +1156,7 @@ __global__ void __launch_bounds__((S4FCfg<T, N>::TPB))
     const int n2 = r0 + d * RS1;
     cpx<T> w[R1];
 #pragma unroll
-    for (int k1 = 0; k1 < R1; ++k1) w[k1] = buf[(k1 * M + n2) * W + c];
+    for (int k1 = 0; k1 < R1; ++k1) w[k1] = v[d][k1];
 #pragma unroll
     for (int k1 = 1; k1 < R1; ++k1) w[k1] = cmulc(w[k1], tw1[n2 * k1]);
     dft<T, R1, +1>(w);
