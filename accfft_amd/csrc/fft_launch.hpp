@@ -126,18 +126,6 @@ inline bool narrow_tiles(int n) {
   return (mask >> l) & 1u;
 }
 
-// Threads per block of the contiguous-line passes of length 128 and 256
-// (128, 256 or 512; ACCFFT_AMD_ZT, read once; default 256).
-template <typename T>
-inline int z_threads(int n) {
-  static const int zt = [] {
-    const char* e = getenv("ACCFFT_AMD_ZT");
-    return e ? atoi(e) : 256;
-  }();
-  (void)n;
-  return zt;
-}
-
 // Strided C2C tiles of 128 KiB of LDS (one workgroup per CU, twice the row
 // segment) for the lengths whose bit log2(n) is set in ACCFFT_AMD_SWIDE.
 inline bool wide_tiles(int n) {
@@ -385,17 +373,6 @@ hipError_t launch_fft(bool strided, int mode, int sign, FftArgs<T> a, const Axis
         per = SCfg<T, NN, 0, 32>::W;                                               \
         a.lpw = per;                                                               \
         return launch_s_n<T, NN, 32>(mode, sign, a, (units + per - 1) / per, 0, st); \
-      }                                                                            \
-      const int zt = z_threads<T>(NN);                                             \
-      if (!strided && zt == 128) {                                                 \
-        per = ZCfg<NN, 0, 128>::LPW;                                               \
-        a.lpw = per;                                                               \
-        return launch_z_n<T, NN, 128>(mode, sign, a, (units + per - 1) / per, 0, st); \
-      }                                                                            \
-      if (!strided && zt == 512) {                                                 \
-        per = ZCfg<NN, 0, 512>::LPW;                                               \
-        a.lpw = per;                                                               \
-        return launch_z_n<T, NN, 512>(mode, sign, a, (units + per - 1) / per, 0, st); \
       }                                                                            \
     }                                                                              \
     per = strided ? SCfg<T, NN>::W : ZCfg<NN>::LPW;                                \
