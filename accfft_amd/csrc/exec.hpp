@@ -98,6 +98,9 @@ inline const char* mode_name(int mode, int sign) {
   }
 }
 
+template <typename F>
+inline int prof_launch(accfft_amd_plan* p, const char* name, double bytes, F&& launch);
+
 // Launches one pass; with profiling on, brackets it with events labelled
 // "<z|s>fft_<mode>_n<N>_<tag>" and records its algorithmic bytes
 // (elements read + elements written, each once).
@@ -136,6 +139,12 @@ inline int run_pass(accfft_amd_plan* p, const char* tag, bool strided, int mode,
   snprintf(name, sizeof(name), "%cfft_%s_n%d_%s_%s%s", strided ? 's' : 'z',
            mode_name<T>(mode, sign), (int)n, tag, sizeof(T) == 4 ? "f32" : "f64",
            split ? sname[p->split & 3] : "");
+  return prof_launch(p, name, bytes, launch);
+}
+
+// Runs launch() between profiling events recorded under `name`.
+template <typename F>
+inline int prof_launch(accfft_amd_plan* p, const char* name, double bytes, F&& launch) {
   int kind = -1;
   for (size_t i = 0; i < p->prof_kinds.size(); ++i)
     if (p->prof_kinds[i].name == name) kind = (int)i;
@@ -389,6 +398,50 @@ struct Ops {
     const int64_t np = (Nh + q - 1) / q * q;
     return (np - Nh) * 50 <= Nh ? np : Nh;
   }
+  // Slab shares with 2048-point y and z axes (FP32): the z pass carries the
+  // first radix-8 level of the y transform (zy_r2c_kernel / zy_c2r_kernel)
+  // and the y pass is ZY_R launches of 256-point FFTs over consecutive
+  // rows, which read and write wide row segments instead of the 64-byte
+  // segments of 2048-row tiles.  ACCFFT_AMD_ZY=0 keeps the separate passes.
+  static bool zy_fused(const accfft_amd_plan* p) {
+    if constexpr (sizeof(T) != 4) {
+      return false;
+    } else {
+      static const bool on = env_int("ACCFFT_AMD_ZY", 1) != 0;
+      const int64_t n1 = p->n[1] / p->nprocs;
+      return on && p->kind == 3 && p->tf_ysub.n == 2048 / ZY_R &&
+             (p->nprocs == 1 || (n1 >= 2 * ZY_R && (n1 & (n1 - 1)) == 0));
+    }
+  }
+  static const AxisTab<T>& ysub(const accfft_amd_plan* p) {
+    if constexpr (sizeof(T) == 4) return p->tf_ysub;
+    else return p->td[1];  // not reached: zy_fused is false for FP64
+  }
+  static int run_zy(accfft_amd_plan* p, int sign, const FftArgs<T>& a) {
+    auto launch = [&]() { return launch_zy<T>(sign, a, tab(p, 1).tw_full, p->stream); };
+    if (!p->prof) return status_of(launch());
+    const double n = (double)p->n[2], nh = (double)(p->n[2] / 2 + 1);
+    const double bytes = (double)a.nlines * ZY_R * (n * sizeof(T) + nh * 2 * sizeof(T));
+    return prof_launch(p, sign < 0 ? "zyfft_r2c_n2048_z_f32" : "zyfft_c2r_n2048_z_f32", bytes,
+                       launch);
+  }
+  static FftArgs<T> zy_args(const accfft_amd_plan* p, const void* in, void* out, int64_t n0,
+                            int64_t idist, int64_t odist) {
+    const int64_t N1 = p->n[1], M = N1 / ZY_R;
+    FftArgs<T> a{};
+    a.in = in;
+    a.out = out;
+    a.idist = idist;
+    a.odist = odist;
+    a.Din = N1 * idist;
+    a.Dout = N1 * odist;
+    a.C = M;
+    a.nlines = n0 * M;
+    a.tw = tab(p, 2).tw_static;
+    a.xgroup = xcd_group_z();
+    return a;
+  }
+
   static int r2c_slab(accfft_amd_plan* p, const void* in, void* out) {
     const int64_t N0 = p->n[0], N1 = p->n[1], N2 = p->n[2], Nh = N2 / 2 + 1;
     const int P = p->nprocs;
@@ -399,6 +452,18 @@ struct Ops {
     if (p->exchange && (st = ensure_ws(p, 1, cb))) return st;
     void* zbuf = p->ws[0];
     void* send = p->exchange ? p->ws[1] : out;
+    if (zy_fused(p)) {
+      const int64_t M = N1 / ZY_R;
+      if ((st = run_zy(p, -1, zy_args(p, in, zbuf, n0, N2, Np)))) return st;
+      for (int k1 = 0; k1 < ZY_R; ++k1) {
+        // rows k1 M + n2 -> rows k1 + ZY_R k2 of the exchange blocks
+        auto ay = sargs((const C*)zbuf + k1 * M * Np, (C*)send + k1 * Nh, n0 * Nh, Nh, N1 * Np,
+                        n1 * Nh, Np, ZY_R * Nh);
+        ay.oshift = P == 1 ? 0 : ilog2(n1) - ilog2(ZY_R);
+        ay.Bout = n0 * n1 * Nh;
+        if ((st = run_pass<T>(p, "y", true, M_C2C, -1, ay, ysub(p), 1))) return st;
+      }
+    } else {
     auto az = zargs(in, zbuf, n0 * N1, N2, Np);
     if ((st = run_pass<T>(p, "z", false, M_R2C, -1, az, tab(p, 2), 0))) return st;
     auto ay = sargs(zbuf, send, n0 * Nh, Nh, N1 * Np, n1 * Nh, Np, Nh);
@@ -406,6 +471,7 @@ struct Ops {
     ay.Bout = n0 * n1 * Nh;
     if (P == 1) ay.oshift = 0;
     if ((st = run_pass<T>(p, "y", true, M_C2C, -1, ay, tab(p, 1), 1))) return st;
+    }
     if (p->exchange && (st = exchange(p, send, out, (int64_t)(n0 * n1 * Nh * sizeof(C)), -1)))
       return st;
     auto ax = sargs(out, out, n1 * Nh, n1 * Nh, 0, 0, n1 * Nh, n1 * Nh);
@@ -430,6 +496,18 @@ struct Ops {
       recv = p->ws[1];
     }
     void* ybuf = p->ws[0];
+    if (zy_fused(p)) {
+      const int64_t M = N1 / ZY_R;
+      for (int k1 = 0; k1 < ZY_R; ++k1) {
+        // rows k1 + ZY_R k2 of the received blocks -> rows k1 M + n2
+        auto ay = sargs((const C*)recv + k1 * Nh, (C*)ybuf + k1 * M * Np, n0 * Nh, Nh, n1 * Nh,
+                        N1 * Np, ZY_R * Nh, Np);
+        ay.ishift = P == 1 ? 0 : ilog2(n1) - ilog2(ZY_R);
+        ay.Bin = n0 * n1 * Nh;
+        if ((st = run_pass<T>(p, "y", true, M_C2C, +1, ay, ysub(p), 4))) return st;
+      }
+      return run_zy(p, +1, zy_args(p, ybuf, out, n0, Np, N2));
+    }
     auto ay = sargs(recv, ybuf, n0 * Nh, Nh, n1 * Nh, N1 * Np, Nh, Np);
     ay.ishift = ilog2(n1);
     ay.Bin = n0 * n1 * Nh;
@@ -589,7 +667,7 @@ struct Ops {
       auto ay = sargs(nullptr, nullptr, n0 * Nh, Nh, N1 * Np, n1 * Nh, Np, Nh);
       ay.oshift = P == 1 ? 0 : ilog2(n1);
       ay.Bout = n0 * n1 * Nh;
-      passes.push_back({1, 1, ay, n0 * N1 * Np, N0 * n1 * Nh});
+      if (!zy_fused(p)) passes.push_back({1, 1, ay, n0 * N1 * Np, N0 * n1 * Nh});
       passes.push_back({0, 2, sargs(nullptr, nullptr, n1 * Nh, n1 * Nh, 0, 0, n1 * Nh, n1 * Nh),
                         N0 * n1 * Nh, 0});
     }

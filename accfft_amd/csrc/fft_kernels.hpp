@@ -1192,6 +1192,140 @@ __global__ void __launch_bounds__((S4FCfg<T, N>::TPB), (MODE == M_DERIV ? S4FCfg
 }
 
 // ====================================================================
+// z pass fused with the first level of a two-level y transform (slab share)
+// ====================================================================
+// N1 = R * M along y.  A block owns the rows y = n2 + M n1 (n1 < R) of one
+// x-plane: the R real z lines as R/2 packed complex FFTs of length NZ in
+// LDS, then, per half-spectrum column k, the R row spectra, their R-point
+// DFT over n1 and the twiddle exp(-2 pi i n2 k1 / N1):
+//   out[x][k1 M + n2][k] = e^{-2 pi i n2 k1/N1} sum_n1 e^{-2 pi i n1 k1/R} Z_{n2 + M n1}[k].
+// The y transform is then finished by M-point FFTs over the consecutive
+// rows k1 M + n2 (n2 < M) of each k1, whose outputs are rows k1 + R k2.
+// (X[k1 + R k2] = sum_n2 e^{-2 pi i n2 k2/M} e^{-2 pi i n2 k1/N1}
+//  sum_n1 x[n2 + M n1] e^{-2 pi i n1 k1/R}.)  Compared with a z pass and a
+// y pass over N1-point columns, the traffic is the same and the y pass
+// reads and writes rows in wide segments.
+// a: in / out, idist / odist (row pitches of in and out), Din / Dout (plane
+// strides), C = M, nlines = planes * M blocks, tw = the z length's static
+// table (read from global memory: LDS holds the tile only); twy: the y
+// length's exp(-2 pi i m / N1).
+template <typename T, int NZ, int R>
+__global__ void __launch_bounds__((R / 2 * sched_minnb<NZ, 0>()))
+    zy_r2c_kernel(FftArgs<T> a, const cpx<T>* __restrict__ twy) {
+  constexpr int NT = sched_minnb<NZ, 0>(), LINES = R / 2, TPB = LINES * NT;
+  constexpr int NH = NZ / 2 + 1, EP = LINES * NZ / TPB, LSP = NZ + NZ / 16;
+  __shared__ cpx<T> buf[LINES * LSP];
+  const int t = threadIdx.x;
+  const int64_t blk = xcd_tile(blockIdx.x, gridDim.x, (unsigned)a.xgroup);
+  const int64_t M = a.C, x = blk / M, n2 = blk % M;
+  const T* src = reinterpret_cast<const T*>(a.in) + x * a.Din + n2 * a.idist;
+  const int64_t pair = 2 * M * a.idist;  // rows n2 + M (2p) and n2 + M (2p + 1)
+  cpx<T> v[EP];
+#pragma unroll
+  for (int j = 0; j < EP; ++j) {
+    const int e = t + j * TPB, p = e / NZ, i = e % NZ;
+    const T* ra = src + p * pair;
+    v[j] = cpx<T>{ra[i], ra[M * a.idist + i]};
+  }
+#pragma unroll
+  for (int j = 0; j < EP; ++j) buf[lds_pad16(t + j * TPB)] = v[j];
+  __syncthreads();
+  {
+    const int line = t / NT, q = t % NT;
+    LineAcc<T, 1, 1> acc{buf + line * LSP};
+    static_fft<T, NZ, NT, -1, 1, 0, LINES>(acc, q, a.tw, buf, NZ);
+  }
+  cpx<T> w[R];
+#pragma unroll
+  for (int k1 = 0; k1 < R; ++k1) w[k1] = twy[n2 * k1];
+  cpx<T>* dst = reinterpret_cast<cpx<T>*>(a.out) + x * a.Dout + n2 * a.odist;
+  const int64_t rstep = M * a.odist;
+  const T h = (T)0.5;
+  for (int k = t; k < NH; k += TPB) {
+    cpx<T> u[R];
+#pragma unroll
+    for (int p = 0; p < LINES; ++p) {
+      const cpx<T> z1 = buf[lds_pad16(p * NZ + k)];
+      const cpx<T> z2 = buf[lds_pad16(p * NZ + (k == 0 ? 0 : NZ - k))];
+      u[2 * p] = cpx<T>{h * (z1.x + z2.x), h * (z1.y - z2.y)};
+      u[2 * p + 1] = cpx<T>{h * (z1.y + z2.y), h * (z2.x - z1.x)};
+    }
+    dft<T, R, -1>(u);
+#pragma unroll
+    for (int k1 = 0; k1 < R; ++k1) dst[k1 * rstep + k] = k1 ? cmul(u[k1], w[k1]) : u[k1];
+  }
+}
+
+// The inverse: rows k1 M + n2 of the y transform's M-point stage (k1 < R)
+// -> twiddles exp(+2 pi i n2 k1 / N1) and R-point inverse DFTs over k1 ->
+// the half spectra of the rows n2 + M n1, packed in pairs into complex
+// lines (imaginary parts of DC and Nyquist dropped, as c2r does) -> NZ-point
+// inverse FFTs -> real rows.  a: in / out, idist / odist, Din / Dout, C = M,
+// nlines, tw as above.
+template <typename T, int NZ, int R>
+__global__ void __launch_bounds__((R / 2 * sched_minnb<NZ, 0>()))
+    zy_c2r_kernel(FftArgs<T> a, const cpx<T>* __restrict__ twy) {
+  constexpr int NT = sched_minnb<NZ, 0>(), LINES = R / 2, TPB = LINES * NT;
+  constexpr int NH = NZ / 2 + 1, EP = LINES * NZ / TPB, LSP = NZ + NZ / 16;
+  __shared__ cpx<T> buf[LINES * LSP];
+  const int t = threadIdx.x;
+  const int64_t blk = xcd_tile(blockIdx.x, gridDim.x, (unsigned)a.xgroup);
+  const int64_t M = a.C, x = blk / M, n2 = blk % M;
+  const cpx<T>* src = reinterpret_cast<const cpx<T>*>(a.in) + x * a.Din + n2 * a.idist;
+  const int64_t rstep = M * a.idist;
+  cpx<T> w[R];
+#pragma unroll
+  for (int k1 = 0; k1 < R; ++k1) w[k1] = twy[n2 * k1];
+  // all of the thread's loads in flight before the first LDS write
+  constexpr int KI = (NH + TPB - 1) / TPB;
+  cpx<T> uu[KI][R];
+#pragma unroll
+  for (int ki = 0; ki < KI; ++ki) {
+    const int k = t + ki * TPB;
+    if (k < NH) {
+#pragma unroll
+      for (int k1 = 0; k1 < R; ++k1) uu[ki][k1] = src[k1 * rstep + k];
+    }
+  }
+#pragma unroll
+  for (int ki = 0; ki < KI; ++ki) {
+    const int k = t + ki * TPB;
+    if (k >= NH) break;
+    cpx<T>(&u)[R] = uu[ki];
+#pragma unroll
+    for (int k1 = 1; k1 < R; ++k1) u[k1] = cmulc(u[k1], w[k1]);
+    dft<T, R, +1>(u);
+#pragma unroll
+    for (int p = 0; p < LINES; ++p) {
+      cpx<T> A = u[2 * p], B = u[2 * p + 1];
+      const bool edge = k == 0 || 2 * k == NZ;
+      if (edge) {
+        A.y = (T)0;
+        B.y = (T)0;
+      }
+      buf[lds_pad16(p * NZ + k)] = cpx<T>{A.x - B.y, A.y + B.x};
+      if (!edge) buf[lds_pad16(p * NZ + NZ - k)] = cpx<T>{A.x + B.y, B.x - A.y};
+    }
+  }
+  __syncthreads();
+  {
+    const int line = t / NT, q = t % NT;
+    LineAcc<T, 1, 1> acc{buf + line * LSP};
+    static_fft<T, NZ, NT, +1, 1, 0, LINES>(acc, q, a.tw, buf, NZ);
+  }
+  T* dst = reinterpret_cast<T*>(a.out) + x * a.Dout + n2 * a.odist;
+  const int64_t pair = 2 * M * a.odist;
+#pragma unroll
+  for (int j = 0; j < EP; ++j) {
+    const int e = t + j * TPB, p = e / NZ, i = e % NZ;
+    const cpx<T> z = buf[lds_pad16(e)];
+    T* ra = dst + p * pair;
+    ra[i] = z.x;
+    ra[M * a.odist + i] = z.y;
+  }
+}
+
+// ====================================================================
 // Element-wise kernels
 // ====================================================================
 // out = symbol(kind) * scale * in over a complex block [osize] at global

@@ -410,6 +410,22 @@ hipError_t launch_fft(bool strided, int mode, int sign, FftArgs<T> a, const Axis
   }
 }
 
+// Fused z pass and first radix-8 level of a 2048-point y transform
+// (zy_r2c_kernel / zy_c2r_kernel, NZ = 2048): one block per (plane, n2).
+constexpr int ZY_R = 8;
+template <typename T>
+hipError_t launch_zy(int sign, const FftArgs<T>& a, const cpx<T>* twy, hipStream_t st) {
+  if constexpr (sizeof(T) != 4) {
+    return hipErrorNotSupported;
+  } else {
+    if (a.nlines <= 0) return hipSuccess;
+    dim3 g((unsigned)a.nlines), b(ZY_R / 2 * sched_minnb<2048, 0>());
+    if (sign < 0) hipLaunchKernelGGL((zy_r2c_kernel<T, 2048, ZY_R>), g, b, 0, st, a, twy);
+    else hipLaunchKernelGGL((zy_c2r_kernel<T, 2048, ZY_R>), g, b, 0, st, a, twy);
+    return hipGetLastError();
+  }
+}
+
 template <typename T>
 hipError_t launch_mult(const cpx<T>* in, cpx<T>* out, int kind, const long N[3],
                        const long os[3], const long ost[3], double scale, hipStream_t st) {

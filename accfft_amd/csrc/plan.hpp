@@ -32,6 +32,10 @@ struct accfft_amd_plan {
   int osize[3] = {0, 0, 0}, ostart[3] = {0, 0, 0};
   afft::AxisTab<float> tf[3];
   afft::AxisTab<double> td[3];
+  // slab share with 2048-point y and z axes (FP32): tables of the
+  // 256-point second level of the y transform (the first radix-8 level
+  // runs inside the z pass); n == 0 when not built
+  afft::AxisTab<float> tf_ysub;
   std::vector<void*> dev_mem;  // table allocations
   void* ws[2] = {nullptr, nullptr};
   size_t ws_bytes[2] = {0, 0};
