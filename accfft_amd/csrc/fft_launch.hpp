@@ -114,7 +114,10 @@ inline void dyn_lds_attrs() {
 // ACCFFT_AMD_SNARROW (read once).  Default: 128 and 256, whose passes are
 // too short to fill the chip twice over with 64 KiB tiles (FP32 256^3:
 // strided passes 38 -> 28 us, round trip 0.206 -> 0.172 ms; 128^3 0.072 ->
-// 0.064 ms; FP64 tiles are the same either way).
+// 0.064 ms; FP64 tiles are the same either way), for passes over at most
+// narrow_max_bytes: on large operands the wider row segments win (the
+// 256-point y passes of the 2048-point slab share: 262 -> 249 us).
+constexpr size_t narrow_max_bytes = (size_t)256 << 20;
 inline bool narrow_tiles(int n) {
   static const unsigned mask = [] {
     const char* e = getenv("ACCFFT_AMD_SNARROW");
@@ -369,7 +372,7 @@ hipError_t launch_fft(bool strided, int mode, int sign, FftArgs<T> a, const Axis
 #define AFFT_CASE(NN)                                                              \
   case NN:                                                                         \
     if constexpr (NN == 128 || NN == 256) {                                        \
-      if (strided && narrow_tiles(NN)) {                                           \
+      if (strided && narrow_tiles(NN) && units * NN * csz <= narrow_max_bytes) {   \
         per = SCfg<T, NN, 0, 32>::W;                                               \
         a.lpw = per;                                                               \
         return launch_s_n<T, NN, 32>(mode, sign, a, (units + per - 1) / per, 0, st); \
