@@ -1207,14 +1207,18 @@ __global__ void __launch_bounds__((S4FCfg<T, N>::TPB), (MODE == M_DERIV ? S4FCfg
 // reads and writes rows in wide segments.
 // a: in / out, idist / odist (row pitches of in and out), Din / Dout (plane
 // strides), C = M, nlines = planes * M blocks, tw = the z length's static
-// table (read from global memory: LDS holds the tile only); twy: the y
-// length's exp(-2 pi i m / N1).
+// table; twy: the y length's exp(-2 pi i m / N1).  The tile (64 KiB at
+// NZ = 2048, R = 8) and the staged z twiddles (16 KiB) fit two blocks per
+// CU only unpadded (with a padded tile and the twiddles read from global
+// memory instead, the twiddle loads held 40 more VGPRs and both kernels
+// ran 6-11 % slower).
 template <typename T, int NZ, int R>
 __global__ void __launch_bounds__((R / 2 * sched_minnb<NZ, 0>()))
     zy_r2c_kernel(FftArgs<T> a, const cpx<T>* __restrict__ twy) {
   constexpr int NT = sched_minnb<NZ, 0>(), LINES = R / 2, TPB = LINES * NT;
-  constexpr int NH = NZ / 2 + 1, EP = LINES * NZ / TPB, LSP = NZ + NZ / 16;
-  __shared__ cpx<T> buf[LINES * LSP];
+  constexpr int NH = NZ / 2 + 1, EP = LINES * NZ / TPB, TWN = sched_twn<NZ, 0>();
+  __shared__ cpx<T> buf[LINES * NZ];
+  __shared__ cpx<T> s_tw[TWN];
   const int t = threadIdx.x;
   const int64_t blk = xcd_tile(blockIdx.x, gridDim.x, (unsigned)a.xgroup);
   const int64_t M = a.C, x = blk / M, n2 = blk % M;
@@ -1227,13 +1231,14 @@ __global__ void __launch_bounds__((R / 2 * sched_minnb<NZ, 0>()))
     const T* ra = src + p * pair;
     v[j] = cpx<T>{ra[i], ra[M * a.idist + i]};
   }
+  stage_twiddles<T, TWN, TPB>(s_tw, a.tw, t);
 #pragma unroll
-  for (int j = 0; j < EP; ++j) buf[lds_pad16(t + j * TPB)] = v[j];
+  for (int j = 0; j < EP; ++j) buf[t + j * TPB] = v[j];
   __syncthreads();
   {
     const int line = t / NT, q = t % NT;
-    LineAcc<T, 1, 1> acc{buf + line * LSP};
-    static_fft<T, NZ, NT, -1, 1, 0, LINES>(acc, q, a.tw, buf, NZ);
+    LineAcc<T, 1> acc{buf + line * NZ};
+    static_fft<T, NZ, NT, -1, 1, 0, LINES>(acc, q, s_tw, buf, NZ);
   }
   cpx<T> w[R];
 #pragma unroll
@@ -1245,8 +1250,8 @@ __global__ void __launch_bounds__((R / 2 * sched_minnb<NZ, 0>()))
     cpx<T> u[R];
 #pragma unroll
     for (int p = 0; p < LINES; ++p) {
-      const cpx<T> z1 = buf[lds_pad16(p * NZ + k)];
-      const cpx<T> z2 = buf[lds_pad16(p * NZ + (k == 0 ? 0 : NZ - k))];
+      const cpx<T> z1 = buf[p * NZ + k];
+      const cpx<T> z2 = buf[p * NZ + (k == 0 ? 0 : NZ - k)];
       u[2 * p] = cpx<T>{h * (z1.x + z2.x), h * (z1.y - z2.y)};
       u[2 * p + 1] = cpx<T>{h * (z1.y + z2.y), h * (z2.x - z1.x)};
     }
@@ -1266,8 +1271,9 @@ template <typename T, int NZ, int R>
 __global__ void __launch_bounds__((R / 2 * sched_minnb<NZ, 0>()))
     zy_c2r_kernel(FftArgs<T> a, const cpx<T>* __restrict__ twy) {
   constexpr int NT = sched_minnb<NZ, 0>(), LINES = R / 2, TPB = LINES * NT;
-  constexpr int NH = NZ / 2 + 1, EP = LINES * NZ / TPB, LSP = NZ + NZ / 16;
-  __shared__ cpx<T> buf[LINES * LSP];
+  constexpr int NH = NZ / 2 + 1, EP = LINES * NZ / TPB, TWN = sched_twn<NZ, 0>();
+  __shared__ cpx<T> buf[LINES * NZ];
+  __shared__ cpx<T> s_tw[TWN];
   const int t = threadIdx.x;
   const int64_t blk = xcd_tile(blockIdx.x, gridDim.x, (unsigned)a.xgroup);
   const int64_t M = a.C, x = blk / M, n2 = blk % M;
@@ -1303,22 +1309,23 @@ __global__ void __launch_bounds__((R / 2 * sched_minnb<NZ, 0>()))
         A.y = (T)0;
         B.y = (T)0;
       }
-      buf[lds_pad16(p * NZ + k)] = cpx<T>{A.x - B.y, A.y + B.x};
-      if (!edge) buf[lds_pad16(p * NZ + NZ - k)] = cpx<T>{A.x + B.y, B.x - A.y};
+      buf[p * NZ + k] = cpx<T>{A.x - B.y, A.y + B.x};
+      if (!edge) buf[p * NZ + NZ - k] = cpx<T>{A.x + B.y, B.x - A.y};
     }
   }
+  stage_twiddles<T, TWN, TPB>(s_tw, a.tw, t);
   __syncthreads();
   {
     const int line = t / NT, q = t % NT;
-    LineAcc<T, 1, 1> acc{buf + line * LSP};
-    static_fft<T, NZ, NT, +1, 1, 0, LINES>(acc, q, a.tw, buf, NZ);
+    LineAcc<T, 1> acc{buf + line * NZ};
+    static_fft<T, NZ, NT, +1, 1, 0, LINES>(acc, q, s_tw, buf, NZ);
   }
   T* dst = reinterpret_cast<T*>(a.out) + x * a.Dout + n2 * a.odist;
   const int64_t pair = 2 * M * a.odist;
 #pragma unroll
   for (int j = 0; j < EP; ++j) {
     const int e = t + j * TPB, p = e / NZ, i = e % NZ;
-    const cpx<T> z = buf[lds_pad16(e)];
+    const cpx<T> z = buf[e];
     T* ra = dst + p * pair;
     ra[i] = z.x;
     ra[M * a.odist + i] = z.y;
