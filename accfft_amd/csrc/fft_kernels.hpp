@@ -655,6 +655,8 @@ struct S4Cfg {
     static constexpr int R1 = R1_, W = W_, TPB = TPB_, KC = KC_, TWF = TWF_, SUB = SUB_, \
                          WPE = WPE_, PB = PB_;                               \
   };
+// 256: only the slab share's 256-point y stage (its plan tables force it)
+AFFT_S4CFG(float, 256, 0, 16, 32, 512, 8, 0, 0, 1, 0)
 AFFT_S4CFG(float, 512, 0, 16, 16, 512, 8, 0, 0, 1, 0)
 AFFT_S4CFG(float, 512, 1, 16, 32, 1024, 8, 0, 0, 1, 0)
 AFFT_S4CFG(float, 512, 2, 16, 16, 512, 8, 1, 0, 1, 0)

@@ -328,6 +328,7 @@ hipError_t launch_fft(bool strided, int mode, int sign, FftArgs<T> a, const Axis
     }
     if (strided && mode == M_C2C && tab.tw_sub && four_step<T>(tab)) {
       switch (n) {
+        case 256: return launch_s4<T, 256>(sign, a, tab, st);
         case 512: return launch_s4<T, 512>(sign, a, tab, st);
         case 1024: return launch_s4<T, 1024>(sign, a, tab, st);
         case 2048: return launch_s4<T, 2048>(sign, a, tab, st);
