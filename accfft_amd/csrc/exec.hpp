@@ -464,13 +464,13 @@ struct Ops {
         if ((st = run_pass<T>(p, "y", true, M_C2C, -1, ay, ysub(p), 1))) return st;
       }
     } else {
-    auto az = zargs(in, zbuf, n0 * N1, N2, Np);
-    if ((st = run_pass<T>(p, "z", false, M_R2C, -1, az, tab(p, 2), 0))) return st;
-    auto ay = sargs(zbuf, send, n0 * Nh, Nh, N1 * Np, n1 * Nh, Np, Nh);
-    ay.oshift = ilog2(n1);
-    ay.Bout = n0 * n1 * Nh;
-    if (P == 1) ay.oshift = 0;
-    if ((st = run_pass<T>(p, "y", true, M_C2C, -1, ay, tab(p, 1), 1))) return st;
+      auto az = zargs(in, zbuf, n0 * N1, N2, Np);
+      if ((st = run_pass<T>(p, "z", false, M_R2C, -1, az, tab(p, 2), 0))) return st;
+      auto ay = sargs(zbuf, send, n0 * Nh, Nh, N1 * Np, n1 * Nh, Np, Nh);
+      ay.oshift = ilog2(n1);
+      ay.Bout = n0 * n1 * Nh;
+      if (P == 1) ay.oshift = 0;
+      if ((st = run_pass<T>(p, "y", true, M_C2C, -1, ay, tab(p, 1), 1))) return st;
     }
     if (p->exchange && (st = exchange(p, send, out, (int64_t)(n0 * n1 * Nh * sizeof(C)), -1)))
       return st;
