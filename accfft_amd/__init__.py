@@ -137,6 +137,46 @@ def _ptr(t) -> Optional[int]:
     return t.data_ptr()
 
 
+def _family(precision: int):
+    import torch
+    return ((torch.float32, torch.complex64) if precision == F32
+            else (torch.float64, torch.complex128))
+
+
+def _arg(t, nbytes: int, precision: int, what: str, device: Optional[int] = None,
+         optional: bool = True) -> Optional[int]:
+    """Device pointer of ``t`` after checking that the kernels may use it:
+    a contiguous GPU tensor (on ``device`` when given) of the precision's
+    real or complex dtype holding at least ``nbytes`` bytes.  ``None`` passes
+    through where the C ABI takes NULL (the plan's own buffers)."""
+    import torch
+    if t is None:
+        if optional:
+            return None
+        raise AccfftError(f"{what}: a tensor is required")
+    if not isinstance(t, torch.Tensor):
+        raise AccfftError(f"{what}: expected a torch.Tensor, got {type(t).__name__}")
+    if not t.is_cuda:
+        raise AccfftError(f"{what}: tensor is on {t.device}, not a GPU")
+    if device is not None and t.device.index != device:
+        raise AccfftError(f"{what}: tensor is on cuda:{t.device.index}, the plan on cuda:{device}")
+    if not t.is_contiguous():
+        raise AccfftError(f"{what}: tensor is not contiguous (pass .contiguous())")
+    if precision is not None and t.dtype not in _family(precision):
+        raise AccfftError(f"{what}: dtype {t.dtype} does not match the plan's precision "
+                          f"({'FP32' if precision == F32 else 'FP64'})")
+    have = t.numel() * t.element_size()
+    if have < nbytes:
+        raise AccfftError(f"{what}: tensor holds {have} bytes, the call needs {nbytes}")
+    return t.data_ptr()
+
+
+def _gpu(t, what: str, nbytes: int = 0):
+    """Checks an argument of the element-wise and copy kernels (any element
+    type): a contiguous GPU tensor of at least ``nbytes`` bytes."""
+    return _arg(t, nbytes, None, what, None, False)
+
+
 def _fp_of(dtype) -> int:
     import torch
     if dtype in (torch.float32, torch.complex64):
@@ -207,6 +247,32 @@ class Plan:
         self.alloc_max = amax.value
         self.inplace = bool(inp.value)
         self._keep = (data, data_out)
+        import torch
+        self.device = torch.cuda.current_device() if torch.cuda.is_available() else None
+        s = 4 if precision == F32 else 8
+        N0, N1, N2 = self.n
+        nh = N2 // 2 + 1
+        if kind == "c2c":
+            self._in_bytes = N0 * N1 * N2 * 2 * s
+        else:
+            z = 2 * nh if self.inplace else self.isize[2]
+            self._in_bytes = self.isize[0] * self.isize[1] * z * s
+        self._out_bytes = self.osize[0] * self.osize[1] * self.osize[2] * 2 * s
+
+    def _a(self, t, nbytes, what, optional=True):
+        return _arg(t, nbytes, self.precision, what, self.device, optional)
+
+    def _field(self, t, what, optional=True):
+        """A real field [N0][N1][N2] of the operators."""
+        s = 4 if self.precision == F32 else 8
+        return self._a(t, self.n[0] * self.n[1] * self.n[2] * s, what, optional)
+
+    def _axis_bytes(self, axis: int) -> int:
+        """Bytes of the half spectrum of a per-axis r2c along ``axis``."""
+        s = 4 if self.precision == F32 else 8
+        m = list(self.n)
+        m[axis] = m[axis] // 2 + 1
+        return m[0] * m[1] * m[2] * 2 * s
 
     @property
     def handle(self):
@@ -244,44 +310,57 @@ class Plan:
 
     def execute_r2c(self, data=None, data_out=None, timer=None, xyz: int = 7):
         arr, tp = self._timer(timer)
-        _check(lib().accfft_amd_execute_r2c(self._bind(), _ptr(data), _ptr(data_out), tp, xyz),
+        out_b = self._out_bytes if xyz & 4 else 0
+        _check(lib().accfft_amd_execute_r2c(self._bind(), self._a(data, self._in_bytes, "r2c input"),
+                                            self._a(data_out, out_b, "r2c output"), tp, xyz),
                "execute_r2c")
         self._untimer(timer, arr)
 
     def execute_c2r(self, data=None, data_out=None, timer=None, xyz: int = 7):
         arr, tp = self._timer(timer)
-        _check(lib().accfft_amd_execute_c2r(self._bind(), _ptr(data), _ptr(data_out), tp, xyz),
+        out_b = self._in_bytes if xyz & 4 else 0
+        _check(lib().accfft_amd_execute_c2r(self._bind(), self._a(data, self._out_bytes, "c2r input"),
+                                            self._a(data_out, out_b, "c2r output"), tp, xyz),
                "execute_c2r")
         self._untimer(timer, arr)
 
     def execute_c2c(self, direction: int, data=None, data_out=None, timer=None, xyz: int = 7):
         arr, tp = self._timer(timer)
-        _check(lib().accfft_amd_execute_c2c(self._bind(), direction, _ptr(data), _ptr(data_out),
+        _check(lib().accfft_amd_execute_c2c(self._bind(), direction,
+                                            self._a(data, self._in_bytes, "c2c input"),
+                                            self._a(data_out, self._out_bytes, "c2c output"),
                                             tp, xyz), "execute_c2c")
         self._untimer(timer, arr)
 
     def execute_r2c_axis(self, axis: int, data, data_out, timer=None):
         arr, tp = self._timer(timer)
-        _check(lib().accfft_amd_execute_r2c_axis(self._bind(), axis, _ptr(data), _ptr(data_out),
-                                                 tp), "execute_r2c_axis")
+        _check(lib().accfft_amd_execute_r2c_axis(
+            self._bind(), axis, self._field(data, "r2c_axis input", False),
+            self._a(data_out, self._axis_bytes(axis), "r2c_axis output", False), tp),
+            "execute_r2c_axis")
         self._untimer(timer, arr)
 
     def execute_c2r_axis(self, axis: int, data, data_out, timer=None):
         arr, tp = self._timer(timer)
-        _check(lib().accfft_amd_execute_c2r_axis(self._bind(), axis, _ptr(data), _ptr(data_out),
-                                                 tp), "execute_c2r_axis")
+        _check(lib().accfft_amd_execute_c2r_axis(
+            self._bind(), axis, self._a(data, self._axis_bytes(axis), "c2r_axis input", False),
+            self._field(data_out, "c2r_axis output", False), tp), "execute_c2r_axis")
         self._untimer(timer, arr)
 
     # operators (include/accfft_operators_gpu.h:44-67)
     def grad(self, A, Ax=None, Ay=None, Az=None, mask: int = 7, timer=None, slow=False):
         arr, tp = self._timer(timer)
-        _check(lib().accfft_amd_grad(self._bind(), _ptr(Ax), _ptr(Ay), _ptr(Az), _ptr(A), mask,
-                                     tp, int(slow)), "grad")
+        _check(lib().accfft_amd_grad(self._bind(), self._field(Ax, "grad x output"),
+                                     self._field(Ay, "grad y output"),
+                                     self._field(Az, "grad z output"),
+                                     self._field(A, "grad input", False), mask, tp, int(slow)),
+               "grad")
         self._untimer(timer, arr)
 
     def _spectral(self, name, out, A, timer):
         arr, tp = self._timer(timer)
-        _check(getattr(lib(), name)(self._bind(), _ptr(out), _ptr(A), tp), name)
+        _check(getattr(lib(), name)(self._bind(), self._field(out, name + " output", False),
+                                    self._field(A, name + " input", False), tp), name)
         self._untimer(timer, arr)
 
     def laplace(self, A, out, timer=None):
@@ -301,14 +380,19 @@ class Plan:
         the Fourier filter (steps/step4/kernels.cu:24-110) fused into the x
         pass: phi = zero-mean solution of laplace(phi) = rho on the unit box."""
         arr, tp = self._timer(timer)
-        _check(lib().accfft_amd_poisson_solve(self._bind(), int(method), _ptr(phi), _ptr(rho), tp),
+        _check(lib().accfft_amd_poisson_solve(self._bind(), int(method),
+                                              self._field(phi, "poisson_solve output", False),
+                                              self._field(rho, "poisson_solve input", False), tp),
                "poisson_solve")
         self._untimer(timer, arr)
 
     def divergence(self, Ax, Ay, Az, out, timer=None, slow=False):
         arr, tp = self._timer(timer)
-        _check(lib().accfft_amd_divergence(self._bind(), _ptr(out), _ptr(Ax), _ptr(Ay), _ptr(Az),
-                                           tp, int(slow)), "divergence")
+        _check(lib().accfft_amd_divergence(self._bind(), self._field(out, "divergence output", False),
+                                           self._field(Ax, "divergence x input"),
+                                           self._field(Ay, "divergence y input"),
+                                           self._field(Az, "divergence z input"), tp, int(slow)),
+               "divergence")
         self._untimer(timer, arr)
 
     def set_exchange(self, fn):
@@ -330,27 +414,38 @@ class Plan:
     def adjoint_r2c(self, G, x, timer=None):
         """x = r2c^T(G): the gradient of <G, r2c(x)> with respect to x."""
         arr, tp = self._timer(timer)
-        _check(lib().accfft_amd_adjoint_r2c(self._bind(), _ptr(G), _ptr(x), tp), "adjoint_r2c")
+        _check(lib().accfft_amd_adjoint_r2c(self._bind(),
+                                            self._a(G, self._out_bytes, "adjoint_r2c input", False),
+                                            self._field(x, "adjoint_r2c output", False), tp),
+               "adjoint_r2c")
         self._untimer(timer, arr)
 
     def adjoint_c2r(self, y, G, timer=None):
         """G = c2r^T(y); ``y`` is not overwritten."""
         arr, tp = self._timer(timer)
-        _check(lib().accfft_amd_adjoint_c2r(self._bind(), _ptr(y), _ptr(G), tp), "adjoint_c2r")
+        _check(lib().accfft_amd_adjoint_c2r(self._bind(), self._field(y, "adjoint_c2r input", False),
+                                            self._a(G, self._out_bytes, "adjoint_c2r output", False),
+                                            tp), "adjoint_c2r")
         self._untimer(timer, arr)
 
     def adjoint_grad(self, gx, gy, gz, out, timer=None):
         """out = grad^T(gx, gy, gz) = -div(gx, gy, gz); None components are skipped."""
         arr, tp = self._timer(timer)
-        _check(lib().accfft_amd_adjoint_grad(self._bind(), _ptr(out), _ptr(gx), _ptr(gy),
-                                             _ptr(gz), tp), "adjoint_grad")
+        _check(lib().accfft_amd_adjoint_grad(self._bind(), self._field(out, "adjoint_grad output", False),
+                                             self._field(gx, "adjoint_grad x input"),
+                                             self._field(gy, "adjoint_grad y input"),
+                                             self._field(gz, "adjoint_grad z input"), tp),
+               "adjoint_grad")
         self._untimer(timer, arr)
 
     def adjoint_divergence(self, v, ox=None, oy=None, oz=None, timer=None):
         """(ox, oy, oz) = div^T(v) = -grad(v); None outputs are skipped."""
         arr, tp = self._timer(timer)
-        _check(lib().accfft_amd_adjoint_divergence(self._bind(), _ptr(ox), _ptr(oy), _ptr(oz),
-                                                   _ptr(v), tp), "adjoint_divergence")
+        _check(lib().accfft_amd_adjoint_divergence(
+            self._bind(), self._field(ox, "adjoint_divergence x output"),
+            self._field(oy, "adjoint_divergence y output"),
+            self._field(oz, "adjoint_divergence z output"),
+            self._field(v, "adjoint_divergence input", False), tp), "adjoint_divergence")
         self._untimer(timer, arr)
 
     def tuning(self):
@@ -505,7 +600,11 @@ accfft_inv_laplace_gpuf = accfft_inv_laplace_gpu
 # ------------------------------------------------------------------ element-wise / transposes
 def mult_wave_number(kind: int, wA, A, n, osize, ostart, scale_xyz: int = 7):
     """The extern "C" multipliers of src/operators_cuda.cu:421-486."""
-    _check(lib().accfft_amd_mult_wave_number(kind, _fp_of(A.dtype), _ptr(wA), _ptr(A), _i3(*n),
+    nb = osize[0] * osize[1] * osize[2] * 2 * (4 if _fp_of(A.dtype) == F32 else 8)
+    if wA.dtype != A.dtype:
+        raise AccfftError(f"mult_wave_number: dtypes {wA.dtype} and {A.dtype} differ")
+    _check(lib().accfft_amd_mult_wave_number(kind, _fp_of(A.dtype), _gpu(wA, "wA", nb),
+                                             _gpu(A, "A", nb), _i3(*n),
                                              _i3(*osize), _i3(*ostart), scale_xyz, _stream()),
            "mult_wave_number")
 
@@ -513,40 +612,48 @@ def mult_wave_number(kind: int, wA, A, n, osize, ostart, scale_xyz: int = 7):
 def poisson_fourier_filter_gpu(data_hat, N, isize, istart, method: int = 0):
     """poisson_fourier_filter_gpu(data_hat, N, isize, istart, methodNb),
     steps/step4/kernels.cu:113-146, in place on a half spectrum block."""
-    _check(lib().accfft_amd_poisson_filter(_fp_of(data_hat.dtype), _ptr(data_hat), _i3(*N),
+    nb = isize[0] * isize[1] * isize[2] * 2 * (4 if _fp_of(data_hat.dtype) == F32 else 8)
+    _check(lib().accfft_amd_poisson_filter(_fp_of(data_hat.dtype), _gpu(data_hat, "data_hat", nb),
+                                           _i3(*N),
                                            _i3(*isize), _i3(*istart), int(method), _stream()),
            "poisson_filter")
 
 
 def axpy(alpha: float, x, y):
     """daxpy_gpu_c(f), src/operators_cuda.cu:470-474: y += alpha * x."""
-    _check(lib().accfft_amd_axpy(_fp_of(x.dtype), x.numel(), alpha, _ptr(x), _ptr(y), _stream()),
+    if x.dtype != y.dtype or y.numel() < x.numel():
+        raise AccfftError(f"axpy: y ({y.dtype}, {y.numel()}) cannot take x ({x.dtype}, {x.numel()})")
+    _check(lib().accfft_amd_axpy(_fp_of(x.dtype), x.numel(), alpha, _gpu(x, "x"), _gpu(y, "y"),
+                                 _stream()),
            "axpy")
 
 
 def local_transpose(r, c, t, inp, out):
     """local_transpose_cuda(r,c,t,in,out), src/transpose_cuda.cu:242-257."""
-    _check(lib().accfft_amd_local_transpose(r, c, t, _ptr(inp), _ptr(out), inp.element_size(),
+    nb = r * c * t * inp.element_size()
+    _check(lib().accfft_amd_local_transpose(r, c, t, _gpu(inp, "in", nb), _gpu(out, "out", nb),
+                                            inp.element_size(),
                                             _stream()), "local_transpose")
 
 
 def local_transpose_ragged(r, c, t, t2, inp, out):
     """local_transpose_cuda(r,c,t,t2,in,out), src/transpose_cuda.cu:207-224."""
-    _check(lib().accfft_amd_local_transpose_ragged(r, c, t, t2, _ptr(inp), _ptr(out),
+    _check(lib().accfft_amd_local_transpose_ragged(r, c, t, t2, _gpu(inp, "in"), _gpu(out, "out"),
                                                    inp.element_size(), _stream()),
            "local_transpose_ragged")
 
 
 def local_transpose_col(r, c, t, t2, inp, out):
     """local_transpose_col_cuda(r,c,t,t2,in,out), src/transpose_cuda.cu:78-95."""
-    _check(lib().accfft_amd_local_transpose_col(r, c, t, t2, _ptr(inp), _ptr(out),
+    _check(lib().accfft_amd_local_transpose_col(r, c, t, t2, _gpu(inp, "in"), _gpu(out, "out"),
                                                 inp.element_size(), _stream()),
            "local_transpose_col")
 
 
 def local_transpose_inplace(r, c, t, A):
     """local_transpose_cuda(r,c,t,A), src/transpose_cuda.cu:227-239."""
-    _check(lib().accfft_amd_local_transpose_inplace(r, c, t, _ptr(A), A.element_size(),
+    _check(lib().accfft_amd_local_transpose_inplace(r, c, t, _gpu(A, "A", r * c * t * A.element_size()),
+                                                    A.element_size(),
                                                     _stream()), "local_transpose_inplace")
 
 
@@ -559,7 +666,8 @@ def memcpy_v1_h1(nprocs_1, howmany, local_n0, n_tuples, local_n1_proc, send_recv
                  N1, local_1_start_proc):
     """Pencil pack, src/transpose_cuda.cu:97-143 (see accfft_amd_pencil_pack)."""
     _check(lib().accfft_amd_pencil_pack(nprocs_1, howmany, local_n0, n_tuples,
-                                        _i64arr(local_n1_proc), _ptr(send_recv), _ptr(data), idist,
+                                        _i64arr(local_n1_proc), _gpu(send_recv, "send_recv"),
+                                        _gpu(data, "data"), idist,
                                         N1, _i64arr(local_1_start_proc), data.element_size(),
                                         _stream()), "pencil_pack")
 
@@ -568,6 +676,7 @@ def memcpy_v1_h2(nprocs_0, howmany, local_0_start_proc, local_n0_proc, data, odi
                  n_tuples, send_recv):
     """Pencil unpack, src/transpose_cuda.cu:145-197 (see accfft_amd_pencil_unpack)."""
     _check(lib().accfft_amd_pencil_unpack(nprocs_0, howmany, _i64arr(local_0_start_proc),
-                                          _i64arr(local_n0_proc), _ptr(data), odist, local_n1,
-                                          n_tuples, _ptr(send_recv), data.element_size(),
+                                          _i64arr(local_n0_proc), _gpu(data, "data"), odist,
+                                          local_n1, n_tuples, _gpu(send_recv, "send_recv"),
+                                          data.element_size(),
                                           _stream()), "pencil_unpack")

@@ -186,7 +186,9 @@ static void step2_fp64(int* n) {
   (void)hipFree(data);
 }
 
-static void step3_fp64(int* n) {
+// step3 with and without INPLACE (steps/step3/step3_gpu.cpp:52-54, 61-62,
+// 83-84, 99-100): the in-place build plans and executes with data_hat = data
+static void step3_fp64(int* n, bool inplace) {
   MPI_Comm c_comm;
   int c_dims[2] = {1, 1}, isize[3], osize[3], istart[3], ostart[3];
   accfft_create_comm(MPI_COMM_WORLD, c_dims, &c_comm);
@@ -199,17 +201,67 @@ static void step3_fp64(int* n) {
   }
   Complex *data, *data_hat;
   (void)hipMalloc((void**)&data, alloc_max);
-  (void)hipMalloc((void**)&data_hat, alloc_max);
+  if (inplace) data_hat = data;
+  else (void)hipMalloc((void**)&data_hat, alloc_max);
   (void)hipMemcpy(data, z.data(), 2 * N * sizeof(double), hipMemcpyHostToDevice);
   accfft_plan_gpu* plan = accfft_plan_dft_3d_c2c_gpu(n, data, data_hat, c_comm, ACCFFT_MEASURE);
   accfft_execute_c2c_gpu(plan, ACCFFT_FORWARD, data, data_hat);
+  // spectrum of the forward transform against a direct DFT at two bins
+  std::vector<double> h = to_host((double*)data_hat, 2 * N);
+  double worst = 0, scale = 0;
+  for (size_t l = 0; l < N; ++l) scale = fmax(scale, fabs(h[2 * l]));
+  const int bins[2][3] = {{0, 0, 0}, {1, 2, 3 % n[2]}};
+  for (int b = 0; b < 2; ++b) {
+    double re = 0, im = 0;
+    for (int i = 0; i < n[0]; ++i)
+      for (int j = 0; j < n[1]; ++j)
+        for (int k = 0; k < n[2]; ++k) {
+          const double ph = -2 * M_PI * ((double)bins[b][0] * i / n[0] +
+                                          (double)bins[b][1] * j / n[1] +
+                                          (double)bins[b][2] * k / n[2]);
+          const size_t l = ((size_t)i * n[1] + j) * n[2] + k;
+          re += z[2 * l] * cos(ph) - z[2 * l + 1] * sin(ph);
+          im += z[2 * l] * sin(ph) + z[2 * l + 1] * cos(ph);
+        }
+    const size_t l = ((size_t)bins[b][0] * n[1] + bins[b][1]) * n[2] + bins[b][2];
+    worst = fmax(worst, fmax(fabs(h[2 * l] - re), fabs(h[2 * l + 1] - im)));
+  }
   accfft_execute_c2c_gpu(plan, ACCFFT_BACKWARD, data_hat, data);
   std::vector<double> b = to_host((double*)data, 2 * N);
   for (double& v : b) v /= (double)N;
-  report("step3 c2c FP64 round trip", rel_l1(b, z), 1e-10);
+  report(inplace ? "step3 in-place c2c FP64 round trip" : "step3 c2c FP64 round trip",
+         rel_l1(b, z), 1e-10);
+  report(inplace ? "step3 in-place c2c FP64 DFT bins" : "step3 c2c FP64 DFT bins",
+         worst / scale, 1e-10);
   accfft_destroy_plan(plan);
   (void)hipFree(data);
-  (void)hipFree(data_hat);
+  if (!inplace) (void)hipFree(data_hat);
+}
+
+// the FP32 twin of the in-place build (steps/step3/step3_gpuf.cpp)
+static void step3_fp32_inplace(int* n) {
+  MPI_Comm c_comm;
+  int c_dims[2] = {1, 1}, isize[3], osize[3], istart[3], ostart[3];
+  accfft_create_comm(MPI_COMM_WORLD, c_dims, &c_comm);
+  const int alloc_max = accfft_local_size_dft_c2c_gpuf(n, isize, istart, osize, ostart, c_comm);
+  const size_t N = (size_t)n[0] * n[1] * n[2];
+  std::vector<double> a = sample(n, gauss4), z(2 * N);
+  for (size_t i = 0; i < N; ++i) {
+    z[2 * i] = a[i];
+    z[2 * i + 1] = 0.5 * a[i];
+  }
+  std::vector<float> zf(z.begin(), z.end());
+  Complexf* data;
+  (void)hipMalloc((void**)&data, alloc_max);
+  (void)hipMemcpy(data, zf.data(), 2 * N * sizeof(float), hipMemcpyHostToDevice);
+  accfft_plan_gpuf* plan = accfft_plan_dft_3d_c2c_gpuf(n, data, data, c_comm, ACCFFT_MEASURE);
+  accfft_execute_c2c_gpuf(plan, ACCFFT_FORWARD, data, data);
+  accfft_execute_c2c_gpuf(plan, ACCFFT_BACKWARD, data, data);
+  std::vector<double> b = to_host((float*)data, 2 * N);
+  for (double& v : b) v /= (double)N;
+  report("step3 in-place c2c FP32 round trip", rel_l1(b, z), 1e-5);
+  accfft_destroy_plan(plan);
+  (void)hipFree(data);
 }
 
 // ------------------------------------------------------------------ step4
@@ -384,7 +436,11 @@ int main(int argc, char** argv) {
   step1_fp32(n1);
   int n2[3] = {32, 48, 30};
   step2_fp64(n2);
-  step3_fp64(n2);
+  step3_fp64(n2, false);
+  step3_fp64(n2, true);
+  int n3[3] = {64, 512, 32};
+  step3_fp64(n3, true);
+  step3_fp32_inplace(n3);
   int n4[3] = {64, 48, 40};
   step4_fp64(n4);
   step5_fp64(n5);

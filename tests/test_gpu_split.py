@@ -5,8 +5,8 @@ The one-wavefront DFT-16 self-test pins the v_mfma_f32_32x32x16 lane layout
 (a wrong row/column/accumulator mapping gives O(1) errors) and the split
 error of one stage.  The 3-D transforms are then compared with the FP64
 oracle; the measured error of each split form is printed and held to the
-bound its operand precision allows (per stage ~2^-17 for 2-term BF16,
-~2^-22 for 2-term FP16 and 3-term BF16, summed over up to 9 stages).
+drop-in FP32 bound, relative L2 <= 1e-5 (one matrix-core stage per axis:
+per stage ~2^-17 for 2-term BF16, ~2^-22 for 2-term FP16 and 3-term BF16).
 """
 import ctypes
 
@@ -19,7 +19,7 @@ pytestmark = pytest.mark.gpu
 
 SPLITS = {"bf16x2": (1, 0x1000), "bf16x3": (2, 0x2000), "fp16x2": (3, 0x4000)}
 # bound on rel-L2 vs FP64 for a full 3-D transform
-BOUND3D = {"bf16x2": 6e-5, "bf16x3": 1e-5, "fp16x2": 1e-5}
+BOUND3D = {"bf16x2": 1e-5, "bf16x3": 1e-5, "fp16x2": 1e-5}
 
 
 @pytest.mark.parametrize("split", list(SPLITS))
@@ -44,7 +44,7 @@ def test_dft16_selftest(dev, split, sign):
     y = yd.cpu().numpy()
     err = rel_l2(y, ref)
     print(f"dft16 {split} sign {sign}: rel-L2 {err:.3e}")
-    assert err < (3e-5 if split == "bf16x2" else 1e-6)
+    assert err < (1e-5 if split == "bf16x2" else 1e-6)
     # every vector individually (layout: each output lands in its own slot)
     for b in range(32):
         nb = np.linalg.norm(ref[b])
