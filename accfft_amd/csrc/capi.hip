@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <new>
 
 #include "../../include/accfft_amd.h"
@@ -203,6 +204,37 @@ int timed(accfft_amd_plan* p, double* timer, F&& body) {
 
 }  // namespace
 
+// Scratch of accfft_amd_inv_laplace_noalloc: one spectrum of at most
+// ACCFFT_AMD_NOALLOC_MAX_BYTES, a file-scope device array that each GPU
+// holds from the library's load until the process ends.  Host code uses
+// the array only through the device address hipGetSymbolAddress returns
+// for the current device (the name itself is a host-side stand-in).
+static __device__ __align__(256) unsigned char scratch[ACCFFT_AMD_NOALLOC_MAX_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+void* scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current.
+int scratch_address(int device, void** out) {
+  if (device < 0 || device >= kMaxDevices) return ACCFFT_AMD_ERR_INVALID;
+  std::lock_guard<std::mutex> g(scratch_mu);
+  if (!scratch_dev[device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      (void)hipGetLastError();
+      return ACCFFT_AMD_ERR_HIP;
+    }
+    scratch_dev[device] = ptr;
+  }
+  *out = scratch_dev[device];
+  return ACCFFT_AMD_OK;
+}
+}  // namespace
+
 extern "C" {
 
 const char* accfft_amd_status_string(int s) {
@@ -213,6 +245,8 @@ const char* accfft_amd_status_string(int s) {
     case ACCFFT_AMD_ERR_UNSUPPORTED: return "unsupported transform length";
     case ACCFFT_AMD_ERR_NOT_BAKED: return "plan of the wrong kind for this call";
     case ACCFFT_AMD_ERR_HIP: return "HIP runtime error";
+    case ACCFFT_AMD_ERR_SCRATCH:
+      return "workspace larger than the fixed scratch of the no-allocation entry point";
     case ACCFFT_AMD_ERR_ALLOC: return "device allocation failed";
     case ACCFFT_AMD_ERR_NULL: return "NULL plan or pointer";
     default: return "unknown status";
@@ -458,6 +492,28 @@ int accfft_amd_biharmonic(accfft_amd_plan* p, void* BA, const void* A, double* t
 int accfft_amd_inv_laplace(accfft_amd_plan* p, void* out, const void* A, double* timer) {
   return spectral(p, afft::SYM_INV_LAPLACE, out, A, timer);
 }
+int accfft_amd_inv_laplace_noalloc(accfft_amd_plan* p, void* out, const void* A, double* timer) {
+  if (!p || !A || !out) return ACCFFT_AMD_ERR_NULL;
+  if (p->kind != 1) return ACCFFT_AMD_ERR_NOT_BAKED;
+  const int64_t cb = (int64_t)p->n[0] * p->n[1] * (p->n[2] / 2 + 1) * 2 * elem_of(p->fp);
+  if (cb > (int64_t)ACCFFT_AMD_NOALLOC_MAX_BYTES) return ACCFFT_AMD_ERR_SCRATCH;
+  int device = -1;
+  if (hipGetDevice(&device) != hipSuccess) return ACCFFT_AMD_ERR_HIP;
+  if (device != p->device) return ACCFFT_AMD_ERR_INVALID;
+  void* ws = nullptr;
+  int st = scratch_address(device, &ws);
+  if (st) return st;
+  // per-launch profiling would create events; it is suspended for this call
+  const bool prof = p->prof;
+  p->prof = false;
+  st = timed(p, timer, [&] {
+    return p->fp == ACCFFT_AMD_F32 ? afft_spectral_ws_f32(p, afft::SYM_INV_LAPLACE, out, A, ws)
+                                   : afft_spectral_ws_f64(p, afft::SYM_INV_LAPLACE, out, A, ws);
+  });
+  p->prof = prof;
+  return st;
+}
+
 int accfft_amd_inv_biharmonic(accfft_amd_plan* p, void* out, const void* A, double* timer) {
   return spectral(p, afft::SYM_INV_BIHARMONIC, out, A, timer);
 }

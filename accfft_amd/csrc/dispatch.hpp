@@ -16,6 +16,8 @@ struct accfft_amd_plan;
   int afft_grad_##SUF(accfft_amd_plan* p, void* Ax, void* Ay, void* Az, const void* A,      \
                       unsigned mask, int slow);                                             \
   int afft_spectral_##SUF(accfft_amd_plan* p, int sym, void* out, const void* A);           \
+  int afft_spectral_ws_##SUF(accfft_amd_plan* p, int sym, void* out, const void* A,         \
+                             void* ws);                                                     \
   int afft_div_##SUF(accfft_amd_plan* p, void* div, const void* Ax, const void* Ay,         \
                      const void* Az, int slow);                                             \
   int afft_mult_##SUF(int kind, void* wA, const void* A, const int* n, const int* osize,    \

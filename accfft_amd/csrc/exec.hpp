@@ -602,7 +602,13 @@ struct Ops {
     const int64_t N0 = p->n[0], N1 = p->n[1], N2 = p->n[2], Nh = N2 / 2 + 1;
     int st;
     if ((st = ensure_ws(p, 0, (size_t)N0 * N1 * Nh * sizeof(C)))) return st;
-    void* w = p->ws[0];
+    return spectral_in(p, sym, out, A, p->ws[0]);
+  }
+  // the same passes over a caller-provided spectrum w of N0 N1 (N2/2+1)
+  // complex values (every element is written before it is read)
+  static int spectral_in(accfft_amd_plan* p, int sym, void* out, const void* A, void* w) {
+    const int64_t N0 = p->n[0], N1 = p->n[1], N2 = p->n[2], Nh = N2 / 2 + 1;
+    int st;
     if ((st = z_r2c(p, A, w, false))) return st;
     if ((st = y_c2c(p, w, w, Nh, -1))) return st;
     if ((st = x_mult(p, w, w, Nh, sym, 1.0 / ((double)N0 * N1 * N2)))) return st;

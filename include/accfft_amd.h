@@ -40,8 +40,14 @@ enum {
   ACCFFT_AMD_ERR_NOT_BAKED = 4,   /* plan of the wrong kind for this call */
   ACCFFT_AMD_ERR_HIP = 5,         /* a HIP runtime call failed */
   ACCFFT_AMD_ERR_ALLOC = 6,       /* device allocation failed */
-  ACCFFT_AMD_ERR_NULL = 7         /* NULL plan or pointer */
+  ACCFFT_AMD_ERR_NULL = 7,        /* NULL plan or pointer */
+  ACCFFT_AMD_ERR_SCRATCH = 8      /* workspace above ACCFFT_AMD_NOALLOC_MAX_BYTES */
 };
+
+/* Size of the fixed device scratch of accfft_amd_inv_laplace_noalloc: one
+ * half spectrum, N0*N1*(N2/2+1) complex values (160 MiB: 256^3 in FP32 or
+ * FP64). */
+#define ACCFFT_AMD_NOALLOC_MAX_BYTES (160u * 1024u * 1024u)
 
 enum { ACCFFT_AMD_F64 = 0, ACCFFT_AMD_F32 = 1 };
 
@@ -180,6 +186,19 @@ int accfft_amd_biharmonic(accfft_amd_plan* plan, void* BA, const void* A, double
 /* Inverse Laplacian: CPU-only upstream (src/operators.txx:276-333,
  * :1270-1321); here on the GPU. */
 int accfft_amd_inv_laplace(accfft_amd_plan* plan, void* out, const void* A, double* timer);
+/* accfft_amd_inv_laplace without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight solver loops, harnesses that forbid
+ * hipMalloc).  Same kernels and result as accfft_amd_inv_laplace, but the
+ * intermediate spectrum lives in a fixed file-scope device array of
+ * ACCFFT_AMD_NOALLOC_MAX_BYTES per GPU instead of the plan's workspace;
+ * plans whose spectrum is larger get ACCFFT_AMD_ERR_SCRATCH.  The array is
+ * shared by every call on a GPU (and keeps the last call's contents), so
+ * calls that overlap on one GPU -- on different streams or host threads --
+ * race; order them on one stream.  The current device must be the plan's.
+ * Per-launch profiling is not recorded for this call. */
+int accfft_amd_inv_laplace_noalloc(accfft_amd_plan* plan, void* out, const void* A,
+                                   double* timer);
 /* Inverse biharmonic: CPU-only upstream (src/operators.txx:336-392). */
 int accfft_amd_inv_biharmonic(accfft_amd_plan* plan, void* out, const void* A, double* timer);
 /* FFT Poisson solve of the step4 program (steps/step4/step4_gpu.cpp:520-640:
