@@ -10,8 +10,14 @@ Prints ONE JSON line.  ``value`` is milliseconds per round trip (lower is
 better), the time form of the metric; ``roofline`` gives the dominant
 kernel's achieved algorithmic GB/s against the MI355X HBM peak, measured
 with HIP events around each launch on the plan's stream during a timed
-pass; ``cpu_baseline`` times the FP64 CPU oracle (oracle/oracle.c, OpenMP)
-on the host on one round trip of the same shape.
+pass, and its HBM bytes per launch from the committed PMC summary when that
+summary was made from the current kernel sources; ``cpu_baseline`` gives
+the host's CPU timings (median of 5 after 2 warm-ups): the FP64 CPU oracle
+(oracle/oracle.c, OpenMP) on the same round trip, and torch.fft on the CPU
+in FP32 / FP64 at 64^3 .. 512^3 (FFTW, the reference's CPU library, is
+probed and absent on this image).  The split-precision leg runs the
+workload's three seeded inputs and reports each form's rel-L2 and max-abs
+error against the FP64 oracle.
 """
 from __future__ import annotations
 
@@ -36,20 +42,43 @@ def alg_bytes_roundtrip(n, s):
     return 2 * R + 10 * C, 2 * (R + C)
 
 
+ROOT = os.path.dirname(os.path.abspath(__file__))
+
+
+def kernel_sources_sha256():
+    """Hash of the HIP sources (accfft_amd/csrc, sorted by name): a PMC
+    summary stamped with another hash was made from other kernels."""
+    import hashlib
+    h = hashlib.sha256()
+    d = os.path.join(ROOT, "accfft_amd", "csrc")
+    for name in sorted(os.listdir(d)):
+        if name.endswith((".hip", ".hpp", ".inc", ".h")):
+            h.update(name.encode())
+            with open(os.path.join(d, name), "rb") as f:
+                h.update(f.read())
+    return h.hexdigest()
+
+
 def profiled_traffic(kernel, n):
     """HBM bytes per launch of ``kernel`` from the committed rocprofv3 PMC
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by kernels of known size), or
-    None when no summary covers it."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        f"fft_passes_{n[0]}cubed_fp32_hbm_traffic.md")
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by kernels of known size) and
+    where they come from.  The bytes are None when no summary covers the
+    kernel or the summary's source stamp differs from the current kernel
+    sources (the counters would describe other code)."""
+    path = os.path.join(ROOT, "profiles", f"fft_passes_{n[0]}cubed_fp32_hbm_traffic.md")
+    src = {"file": os.path.relpath(path, ROOT), "sources_sha256": None, "current": False}
     if n[0] != n[1] or n[1] != n[2] or not os.path.exists(path):
-        return None
+        return None, src
+    val = None
     with open(path) as f:
         for line in f:
             parts = line.split()
+            if len(parts) == 2 and parts[0] == "sources_sha256:":
+                src["sources_sha256"] = parts[1]
             if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch:" and parts[1] == kernel:
-                return int(parts[2])
-    return None
+                val = int(parts[2])
+    src["current"] = src["sources_sha256"] == kernel_sources_sha256()
+    return (val if src["current"] else None), src
 
 
 def run_gpu(n, fp, steps, warmup, profile=True):
@@ -118,20 +147,32 @@ def run_gpu(n, fp, steps, warmup, profile=True):
 
 
 SPLIT_FLAGS = {"bf16x2": 0x1000, "bf16x3": 0x2000, "fp16x2": 0x4000}
+SPLIT_DISTS = ("normal", "band_limited", "dynamic_range")
+
+
+def split_inputs(n, dist):
+    """The split-precision workload's seeded inputs (accfft_amd/fields.py):
+    a real field for the r2c and a complex one for the c2c, FP32."""
+    import numpy as np
+    from accfft_amd import fields
+    make = {"normal": fields.normal, "dynamic_range": fields.dynamic_range,
+            "band_limited": lambda m, s: fields.band_limited(m, s, slope=-5.0 / 3.0)}[dist]
+    seed = fields.SEED0 + 2
+    x = make(n, seed).astype(np.float32)
+    z = fields.complex_of(make, n, seed).astype(np.complex64)
+    return x, z
 
 
 def run_split(n, steps):
     """512^3 FP32 forward c2c and r2c: VALU butterflies vs split-precision
-    matrix-core radix-16 stages.  Errors are rel-L2 against the FP64 path of
-    this library on the same (FP32-rounded) input; the tests hold that path
-    to <= 1e-12 of the FP64 oracle."""
+    matrix-core radix-16 stages, timed on the N(0,1) input.  Every form's
+    output on each of the workload's three inputs is kept on the GPU so that
+    the CPU leg can check it against the FP64 oracle (rel-L2 and max-abs)."""
     import torch
     import accfft_amd as af
 
     dev = torch.device("cuda:0")
-    g = torch.Generator(device=dev)
-    g.manual_seed(20261204)
-    out = {}
+    out, outputs = {}, {}
 
     def timeit(fn):
         for _ in range(2):
@@ -146,53 +187,71 @@ def run_split(n, steps):
         torch.cuda.synchronize()
         return e0.elapsed_time(e1) / steps
 
-    def rel(a, b):
-        return (torch.linalg.vector_norm(a.to(b.dtype) - b) / torch.linalg.vector_norm(b)).item()
-
-    # c2c forward: N(0,1) and the large-dynamic-range field N(0,1) * 10^U(-4,4)
-    z = torch.complex(torch.randn(n, generator=g, device=dev), torch.randn(n, generator=g, device=dev))
-    zo = torch.empty_like(z)
-    zref = torch.empty(n, dtype=torch.complex128, device=dev)
-    p64 = af.Plan("c2c", n, zref, zref, af.F64)
-    z64 = z.to(torch.complex128)
-    p64.execute_c2c(-1, z64, zref)
-    mag = torch.pow(10.0, torch.rand(n, generator=g, device=dev) * 8 - 4)
-    zd = z * mag
-    zdref = torch.empty_like(zref)
-    p64.execute_c2c(-1, zd.to(torch.complex128), zdref)
-    del z64
-    # r2c forward
-    x = torch.randn(n, generator=g, device=dev)
+    forms = [("fp32_valu", 0)] + list(SPLIT_FLAGS.items())
     h = torch.empty((n[0], n[1], n[2] // 2 + 1), dtype=torch.complex64, device=dev)
-    href = torch.empty(h.shape, dtype=torch.complex128, device=dev)
-    q64 = af.Plan("r2c", n, x, href, af.F64)
-    q64.execute_r2c(x.to(torch.float64), href)
-    torch.cuda.synchronize()
-    p64.destroy()
-    q64.destroy()
-
-    for name, flag in [("fp32_valu", 0)] + list(SPLIT_FLAGS.items()):
-        pc = af.Plan("c2c", n, z, zo, af.F32, af.ACCFFT_MEASURE | flag)
-        pr = af.Plan("r2c", n, x, h, af.F32, af.ACCFFT_MEASURE | flag)
-        tc = timeit(lambda: pc.execute_c2c(-1, z, zo))
-        e_c = rel(zo, zref)
-        pc.execute_c2c(-1, zd, zo)
-        torch.cuda.synchronize()
-        e_cd = rel(zo, zdref)
-        tr = timeit(lambda: pr.execute_r2c(x, h))
-        e_r = rel(h, href)
-        out[name] = {"c2c_ms": round(tc, 4), "r2c_ms": round(tr, 4), "rel_l2_c2c": e_c,
-                     "rel_l2_c2c_dynrange": e_cd, "rel_l2_r2c": e_r}
-        pc.destroy()
-        pr.destroy()
-        log(f"  {name:10s} c2c {tc:.4f} ms ({e_c:.2e}, dyn {e_cd:.2e})  r2c {tr:.4f} ms ({e_r:.2e})")
+    for dist in SPLIT_DISTS:
+        x, z = split_inputs(n, dist)
+        xd = torch.from_numpy(x).to(dev)
+        zd = torch.from_numpy(z).to(dev)
+        del x, z
+        zo = torch.empty_like(zd)
+        for name, flag in forms:
+            pc = af.Plan("c2c", n, zd, zo, af.F32, af.ACCFFT_MEASURE | flag)
+            pr = af.Plan("r2c", n, xd, h, af.F32, af.ACCFFT_MEASURE | flag)
+            if dist == "normal":
+                tc = timeit(lambda: pc.execute_c2c(-1, zd, zo))
+                tr = timeit(lambda: pr.execute_r2c(xd, h))
+                out[name] = {"c2c_ms": round(tc, 4), "r2c_ms": round(tr, 4)}
+                log(f"  {name:10s} c2c {tc:.4f} ms  r2c {tr:.4f} ms")
+            pc.execute_c2c(-1, zd, zo)
+            pr.execute_r2c(xd, h)
+            torch.cuda.synchronize()
+            outputs[(dist, "c2c", name)] = zo.clone()
+            outputs[(dist, "r2c", name)] = h.clone()
+            pc.destroy()
+            pr.destroy()
+        del xd, zd, zo
     base = out["fp32_valu"]
     for name in SPLIT_FLAGS:
         o = out[name]
         o["speedup_c2c_vs_fp32"] = round(base["c2c_ms"] / o["c2c_ms"], 4)
         o["speedup_r2c_vs_fp32"] = round(base["r2c_ms"] / o["r2c_ms"], 4)
-        o["meets_rel_err_1e-5"] = max(o["rel_l2_c2c"], o["rel_l2_c2c_dynrange"], o["rel_l2_r2c"]) <= 1e-5
-    return out
+    return out, outputs
+
+
+def check_split(n, split, outputs):
+    """rel-L2 and max-abs error of every precision form against the FP64
+    oracle (oracle/oracle.c) on the workload's three inputs; the oracle is
+    the checker here, its outputs are compared, not timed."""
+    import numpy as np
+    import torch
+    import oracle
+
+    oracle.build()
+    dev = torch.device("cuda:0")
+    for dist in SPLIT_DISTS:
+        x, z = split_inputs(n, dist)
+        for kind in ("r2c", "c2c"):
+            ref = (oracle.r2c(x.astype(np.float64)) if kind == "r2c"
+                   else oracle.c2c(z.astype(np.complex128), -1))
+            rd = torch.from_numpy(ref).to(dev)
+            del ref
+            nr = torch.linalg.vector_norm(rd).item()
+            mr = rd.abs().max().item()
+            for name in split:
+                o = outputs.pop((dist, kind, name))
+                d = o.to(torch.complex128) - rd
+                split[name][f"rel_l2_{kind}_{dist}"] = torch.linalg.vector_norm(d).item() / nr
+                split[name][f"max_abs_{kind}_{dist}"] = d.abs().max().item()
+                split[name][f"max_abs_rel_{kind}_{dist}"] = d.abs().max().item() / mr
+                del o, d
+            del rd
+        del x, z
+    for name, o in split.items():
+        worst = max(v for k, v in o.items() if k.startswith("rel_l2_"))
+        o["rel_l2_max"] = worst
+        o["meets_rel_err_1e-5"] = worst <= 1e-5
+        log(f"  {name:10s} worst rel-L2 vs the FP64 oracle {worst:.2e}")
 
 
 def _timeit(fn, steps, warmup=2):
@@ -365,38 +424,99 @@ def run_ops(steps):
     return out
 
 
-def cpu_baseline(n, budget_s=30.0):
-    """FP64 oracle round trip on the host, all allowed threads."""
+def _median_ms(fn, runs=5, warmup=2):
+    """Median wall time of ``runs`` calls after ``warmup`` calls, in ms
+    (the reference's practice: steps/step1/step1_gpu.cpp:57-58)."""
+    import statistics
+    for _ in range(warmup):
+        fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append((time.perf_counter() - t0) * 1e3)
+    return statistics.median(ts)
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _probe_fftw():
+    """The reference's CPU path is FFTW (src/accfft.cpp:179-338): probe for
+    the libraries with dlopen."""
+    import ctypes
+    import ctypes.util
+    found = {}
+    for lib in ("fftw3", "fftw3f", "fftw3_threads", "fftw3f_threads"):
+        path = ctypes.util.find_library(lib)
+        ok = False
+        for cand in ([path] if path else []) + [f"lib{lib}.so.3", f"lib{lib}.so"]:
+            try:
+                ctypes.CDLL(cand)
+                ok = True
+                break
+            except OSError:
+                continue
+        found[lib] = ok
+    return found
+
+
+def cpu_baseline(n, full=False):
+    """CPU timings on the host cores, median of 5 runs after 2 warm-ups.
+
+    The reference's own CPU path (FFTW, 1 MPI rank + OpenMP) cannot run: the
+    libraries are probed and, when absent, the comparator is a library CPU
+    FFT, torch.fft.rfftn + irfftn (pocketfft) in FP32 and FP64, at 64^3,
+    256^3, 512^3 (and 1024^3 FP32 with ``full``).  ``value`` is this
+    build's FP64 oracle (oracle/oracle.c, OpenMP) on the headline's 512^3
+    round trip, the "port" the contract names."""
     import numpy as np
+    import torch
     import oracle
 
-    oracle.build()
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    threads = min(threads, 16)
-    oracle.set_threads(threads)
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except AttributeError:
+        threads = os.cpu_count() or 1
+    threads = max(1, min(threads, 16))
+    fftw = _probe_fftw()
+    torch.set_num_threads(threads)
     rng = np.random.default_rng(7)
-    # probe at 1/8 of the volume, then decide the sample
-    probe = (n[0] // 2, n[1] // 2, n[2] // 2)
-    a = rng.standard_normal(probe)
-    t0 = time.perf_counter()
-    h = oracle.r2c(a)
-    oracle.c2r(h, probe)
-    tp = time.perf_counter() - t0
-    est_full = tp * 8 * 1.15
-    if est_full <= budget_s:
-        a = rng.standard_normal(n)
-        t0 = time.perf_counter()
-        h = oracle.r2c(a)
-        oracle.c2r(h, n)
-        t = time.perf_counter() - t0
-        sample = f"1 FP64 r2c+c2r round trip of {n[0]}x{n[1]}x{n[2]} (oracle/oracle.c, OpenMP)"
-        ms = t * 1e3
-    else:
-        ms = est_full * 1e3
-        sample = (f"1 FP64 round trip of {probe[0]}x{probe[1]}x{probe[2]}, scaled x8.0x1.15 "
-                  f"to {n[0]}x{n[1]}x{n[2]}")
-    return {"value": round(ms, 3), "unit": "ms per round trip", "cores": threads,
-            "kind": "port", "sample": sample}
+    lib = {}
+    sizes = [64, 256, 512] + ([1024] if full else [])
+    for m in sizes:
+        for dt, nm in ((torch.float32, "fp32"), (torch.float64, "fp64")):
+            if m == 1024 and dt == torch.float64:
+                continue
+            a = torch.from_numpy(rng.standard_normal((m, m, m))).to(dt)
+            t = _median_ms(lambda: torch.fft.irfftn(torch.fft.rfftn(a), s=a.shape))
+            lib[f"{m}^3_{nm}_ms"] = round(t, 3)
+            log(f"  cpu torch.fft {m}^3 {nm} round trip {t:.2f} ms")
+            del a
+    oracle.build()
+    oracle.set_threads(threads)
+    a = rng.standard_normal(n)
+    t_port = _median_ms(lambda: oracle.c2r(oracle.r2c(a), n))
+    log(f"  cpu oracle {n} FP64 round trip {t_port:.1f} ms")
+    return {"value": round(t_port, 3), "unit": "ms per round trip", "cores": threads,
+            "kind": "port",
+            "sample": (f"FP64 r2c+c2r round trip of {n[0]}x{n[1]}x{n[2]} N(0,1) with "
+                       f"oracle/oracle.c (OpenMP, {threads} threads), median of 5 after 2 "
+                       "warm-ups"),
+            "cpu_model": _cpu_model(), "nproc": os.cpu_count(),
+            "fftw_present": fftw,
+            "library_torch_fft_round_trip": lib,
+            "library_note": ("FFTW absent: torch.fft.rfftn+irfftn on the CPU (pocketfft, "
+                             f"{threads} threads), median of 5 after 2 warm-ups"
+                             if not any(fftw.values()) else "FFTW present; not driven")}
 
 
 def main():
@@ -405,7 +525,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=512)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU baseline and the oracle checks of the split leg")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="also time the CPU library FFT at 1024^3")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 side measurement")
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision leg")
     ap.add_argument("--no-wide", action="store_true",
@@ -443,10 +566,10 @@ def main():
                  "fp64_pass_model_frac": round(alg64 / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         log(f"fp64 {n} round trip {ms64:.4f} ms, rel err {err64:.3e}")
 
-    split = None
+    split, split_out = None, None
     if not args.no_split:
         log("split-precision matrix-core stages vs FP32 VALU (forward transforms):")
-        split = run_split(n, max(5, args.steps // 2))
+        split, split_out = run_split(n, max(5, args.steps // 2))
         extra["split_precision"] = split
     if not args.no_wide:
         log("larger shapes:")
@@ -455,6 +578,7 @@ def main():
         log("operators (512^3 FP32), 256^3 round trips, differentiation (256^3 FP32):")
         extra["operators"] = run_ops(max(5, args.steps // 2))
 
+    traffic, traffic_src = profiled_traffic(dname, n)
     out = {
         "metric": "512^3 r2c+c2r ms & %HBM peak; split-BF16 vs FP32 speedup at rel-err<=1e-5",
         "value": round(ms, 4),
@@ -489,21 +613,31 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": profiled_traffic(dname, n),
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "alg_bytes_per_launch": dbytes,
             "avg_launch_us": round(davg_ms * 1e3, 2),
         },
         "cpu_baseline": None,
     }
+    if not args.no_cpu:
+        try:
+            log("cpu baseline:")
+            out["cpu_baseline"] = cpu_baseline(n, args.cpu_full)
+            lib32 = out["cpu_baseline"]["library_torch_fft_round_trip"].get(f"{n[0]}^3_fp32_ms")
+            if lib32:
+                out["config"]["gpu_speedup_vs_cpu_library_fp32"] = round(lib32 / ms, 1)
+        except Exception as e:  # the baseline is reported, never the measured path
+            log(f"cpu baseline failed: {e!r}")
+        if split:
+            log("split-precision errors against the FP64 oracle:")
+            check_split(n, split, split_out)
+    split_out = None
     if split:
         b = split["bf16x2"]
         out["split_bf16_speedup_c2c"] = b["speedup_c2c_vs_fp32"]
-        out["split_bf16_rel_err_max"] = max(b["rel_l2_c2c"], b["rel_l2_c2c_dynrange"], b["rel_l2_r2c"])
-    if not args.no_cpu:
-        try:
-            out["cpu_baseline"] = cpu_baseline(n)
-        except Exception as e:  # the baseline is reported, never the measured path
-            log(f"cpu baseline failed: {e!r}")
+        out["split_bf16_speedup_r2c"] = b["speedup_r2c_vs_fp32"]
+        out["split_bf16_rel_err_max"] = b.get("rel_l2_max")
     print(json.dumps(out), flush=True)
 
 
