@@ -38,8 +38,16 @@ _i3 = ctypes.c_int * 3
 _vp = ctypes.c_void_p
 
 
+ERR_ALLOC = 6  # ACCFFT_AMD_ERR_ALLOC: a device allocation failed
+
+
 class AccfftError(RuntimeError):
-    pass
+    """A failed call; ``status`` is the C ABI's status code (None for
+    argument checks made before the call)."""
+
+    def __init__(self, msg: str, status: Optional[int] = None):
+        super().__init__(msg)
+        self.status = status
 
 
 # int (*)(const void* send, void* recv, int64_t block_bytes, int nprocs,
@@ -123,7 +131,7 @@ def lib() -> ctypes.CDLL:
 def _check(status: int, what: str) -> None:
     if status != OK:
         msg = lib().accfft_amd_status_string(status).decode()
-        raise AccfftError(f"{what}: {msg} (status {status})")
+        raise AccfftError(f"{what}: {msg} (status {status})", status)
 
 
 def _stream():

@@ -18,19 +18,62 @@ differentiated by its own 1-D transform pair).
 No forward saves a tensor for its backward (a linear map's derivative does
 not depend on the point), so a chain of these functions keeps no activations
 alive and recomputation-based checkpointing
-(``torch.utils.checkpoint``) costs nothing extra; ``Spectral.release`` frees
-the plan's intermediate spectra between steps when memory is tight.
+(``torch.utils.checkpoint``) costs nothing extra.  The only intermediate
+spectra are the plans' workspaces, and they are dropped when memory asks
+for it (``set_memory_policy``): after a call when the device's free memory
+is below a low-water mark (or after every call), and, when an allocation
+fails, every plan's workspace and torch's cached blocks are released and
+the call is repeated once.  ``Spectral.release`` drops them by hand.
 """
 from __future__ import annotations
 
-from typing import Dict, Optional, Sequence, Tuple
+import weakref
+from typing import Callable, Dict, Optional, Sequence, Tuple
 
 import torch
 
-from . import ACCFFT_MEASURE, F32, F64, PRESERVE_INPUT, AccfftError, Plan
+from . import ACCFFT_MEASURE, ERR_ALLOC, F32, F64, PRESERVE_INPUT, AccfftError, Plan
 
 __all__ = ["Spectral", "rfft3", "irfft3", "grad", "divergence", "laplace", "inv_laplace",
-           "biharmonic", "inv_biharmonic"]
+           "biharmonic", "inv_biharmonic", "set_memory_policy", "release_all"]
+
+# When the plans' workspaces (the intermediate spectra) are dropped:
+#   low_water: free device memory (bytes) below which a plan releases its
+#              workspace after a call; None disables the check
+#   keep:      False releases after every call (smallest footprint; each
+#              call allocates again)
+_POLICY = {"low_water": 1 << 30, "keep": True}
+_LIVE: "weakref.WeakSet[Spectral]" = weakref.WeakSet()
+
+
+def set_memory_policy(low_water: Optional[int] = 1 << 30, keep: bool = True) -> None:
+    """Sets when intermediate spectra are dropped (see ``_POLICY``)."""
+    if low_water is not None and low_water < 0:
+        raise AccfftError("low_water must be >= 0 or None")
+    _POLICY["low_water"] = low_water
+    _POLICY["keep"] = bool(keep)
+
+
+def release_all() -> None:
+    """Releases the workspace of every live Spectral."""
+    for s in list(_LIVE):
+        s.release()
+
+
+def _retry_on_oom(fn: Callable, *args):
+    """Runs fn; when a device allocation fails (the library's or torch's),
+    drops every plan's workspace and torch's cached blocks and runs it once
+    more."""
+    try:
+        return fn(*args)
+    except AccfftError as e:
+        if e.status != ERR_ALLOC:
+            raise
+    except torch.cuda.OutOfMemoryError:
+        pass
+    release_all()
+    torch.cuda.empty_cache()
+    return fn(*args)
 
 
 class Spectral:
@@ -49,8 +92,27 @@ class Spectral:
         self._ph = (torch.empty(1, dtype=dtype, device=self.device),
                     torch.empty(1, dtype=dtype, device=self.device))
         prec = F32 if dtype == torch.float32 else F64
-        self.plan = Plan("r2c", self.n, self._ph[0], self._ph[1], prec,
-                         ACCFFT_MEASURE | PRESERVE_INPUT | flags)
+        self.plan = _retry_on_oom(lambda: Plan("r2c", self.n, self._ph[0], self._ph[1], prec,
+                                               ACCFFT_MEASURE | PRESERVE_INPUT | flags))
+        self.releases = 0  # workspace releases made by the memory policy
+        _LIVE.add(self)
+
+    def _call(self, fn: Callable, *args):
+        """One plan call under the memory policy."""
+        _retry_on_oom(fn, *args)
+        if not _POLICY["keep"]:
+            self._drop()
+            return
+        lw = _POLICY["low_water"]
+        if lw is not None and self.plan.workspace_bytes() > 0:
+            free, _ = torch.cuda.mem_get_info(self.device)
+            if free < lw:
+                self._drop()
+
+    def _drop(self):
+        if self.plan.workspace_bytes() > 0:
+            self.release()
+            self.releases += 1
 
     @property
     def spectrum_shape(self) -> Tuple[int, int, int]:
@@ -73,55 +135,57 @@ class Spectral:
         return G.contiguous()
 
     def new_real(self):
-        return torch.empty(self.n, dtype=self.dtype, device=self.device)
+        return _retry_on_oom(lambda: torch.empty(self.n, dtype=self.dtype, device=self.device))
 
     def new_spec(self):
-        return torch.empty(self.spectrum_shape, dtype=self.cdtype, device=self.device)
+        return _retry_on_oom(lambda: torch.empty(self.spectrum_shape, dtype=self.cdtype,
+                                                 device=self.device))
 
     # raw (non-differentiable) calls
     def r2c(self, x):
         G = self.new_spec()
-        self.plan.execute_r2c(self._real(x), G)
+        self._call(self.plan.execute_r2c, self._real(x), G)
         return G
 
     def c2r(self, G):
         y = self.new_real()
-        self.plan.execute_c2r(self._spec(G), y)  # PRESERVE_INPUT: G is kept
+        self._call(self.plan.execute_c2r, self._spec(G), y)  # PRESERVE_INPUT: G is kept
         return y
 
     def r2c_adjoint(self, G):
         x = self.new_real()
-        self.plan.adjoint_r2c(self._spec(G), x)
+        self._call(self.plan.adjoint_r2c, self._spec(G), x)
         return x
 
     def c2r_adjoint(self, y):
         G = self.new_spec()
-        self.plan.adjoint_c2r(self._real(y), G)
+        self._call(self.plan.adjoint_c2r, self._real(y), G)
         return G
 
     def grad(self, u):
         outs = [self.new_real() for _ in range(3)]
-        self.plan.grad(self._real(u), *outs)
+        self._call(self.plan.grad, self._real(u), *outs)
         return tuple(outs)
 
     def grad_adjoint(self, gx, gy, gz):
         out = self.new_real()
-        self.plan.adjoint_grad(*(None if g is None else self._real(g) for g in (gx, gy, gz)), out)
+        self._call(self.plan.adjoint_grad,
+                   *(None if g is None else self._real(g) for g in (gx, gy, gz)), out)
         return out
 
     def divergence(self, vx, vy, vz):
         out = self.new_real()
-        self.plan.divergence(self._real(vx), self._real(vy), self._real(vz), out)
+        self._call(self.plan.divergence, self._real(vx), self._real(vy), self._real(vz), out)
         return out
 
     def divergence_adjoint(self, v, need=(True, True, True)):
         outs = [self.new_real() if k else None for k in need]
-        self.plan.adjoint_divergence(self._real(v), *outs)
+        self._call(self.plan.adjoint_divergence, self._real(v), *outs)
         return tuple(outs)
 
     def op(self, name: str, u):
         out = self.new_real()
-        getattr(self.plan, name)(self._real(u), out)
+        self._call(getattr(self.plan, name), self._real(u), out)
         return out
 
 

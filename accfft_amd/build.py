@@ -33,6 +33,9 @@ FLAGS = [
     # no packed FP32 instructions anywhere in the device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
+if os.environ.get("ACCFFT_AMD_SWEEP", "0") not in ("", "0"):
+    # measurement build: tile / order overrides read from the environment
+    FLAGS.append("-DACCFFT_AMD_SWEEP")
 SOURCES = ["fft_f32.hip", "fft_f64.hip", "fft_split.hip", "transpose.hip", "capi.hip"]
 
 

@@ -108,6 +108,7 @@ void free_plan(accfft_amd_plan* p) {
     if (p->ws[i]) (void)hipFree(p->ws[i]);
   if (p->ev0) (void)hipEventDestroy(p->ev0);
   if (p->ev1) (void)hipEventDestroy(p->ev1);
+  if (p->ev_switch) (void)hipEventDestroy(p->ev_switch);
   if (p->evx0) (void)hipEventDestroy(p->evx0);
   if (p->evx1) (void)hipEventDestroy(p->evx1);
   for (auto& q : p->prof_pending) {
@@ -159,7 +160,8 @@ int make_plan(int kind, const int n[3], void* data, void* data_out, int fp, unsi
   p->alloc_max = kind == 1 ? size_r2c(n, p->isize, p->istart, p->osize, p->ostart, 1, 1, 0, 0, elem)
                            : size_c2c(n, p->isize, p->istart, p->osize, p->ostart, 1, 1, 0, 0, elem);
   (void)hipGetDevice(&p->device);
-  if (hipEventCreate(&p->ev0) != hipSuccess || hipEventCreate(&p->ev1) != hipSuccess) {
+  if (hipEventCreate(&p->ev0) != hipSuccess || hipEventCreate(&p->ev1) != hipSuccess ||
+      hipEventCreateWithFlags(&p->ev_switch, hipEventDisableTiming) != hipSuccess) {
     free_plan(p);
     return ACCFFT_AMD_ERR_HIP;
   }
@@ -173,11 +175,25 @@ int make_plan(int kind, const int n[3], void* data, void* data_out, int fp, unsi
   return ACCFFT_AMD_OK;
 }
 
+// Whether the plan holds the tables of a y transform's second level.
+bool ysub_built(const accfft_amd_plan* p) {
+  return p->fp == ACCFFT_AMD_F32 ? p->tf_ysub.n > 0 : p->td_ysub.n > 0;
+}
+
 // Wraps one execute: events around the launches, optional timer / blocking.
 // Events are recorded only when a timer or a blocking call asks for them:
 // each marker costs ~10 us of queue time between kernels.
 template <typename F>
 int timed(accfft_amd_plan* p, double* timer, F&& body) {
+  // Calls of one plan on different streams share its workspaces: a call on
+  // a new stream waits for the work already queued on the previous one.
+  if (p->called && p->stream != p->last_stream) {
+    if (hipEventRecord(p->ev_switch, p->last_stream) != hipSuccess ||
+        hipStreamWaitEvent(p->stream, p->ev_switch, 0) != hipSuccess)
+      return ACCFFT_AMD_ERR_HIP;
+  }
+  p->called = true;
+  p->last_stream = p->stream;
   const bool want = timer || (p->flags & ACCFFT_AMD_BLOCKING);
   if (!want) return body();
   p->timing = timer != nullptr;
@@ -314,11 +330,19 @@ int accfft_amd_plan_get_tuning(const accfft_amd_plan* p, int choice[3]) {
   if (!p || !choice) return ACCFFT_AMD_ERR_NULL;
   for (int ax = 0; ax < 3; ++ax)
     choice[ax] = p->fp == ACCFFT_AMD_F32 ? p->tf[ax].s4v : p->td[ax].s4v;
+  if (p->kind == 1 && ysub_built(p) && !p->yfuse) choice[1] = -3;
   return ACCFFT_AMD_OK;
 }
 
 int accfft_amd_plan_set_tuning(accfft_amd_plan* p, int axis, int choice) {
   if (!p) return ACCFFT_AMD_ERR_NULL;
+  if (p->kind == 1 && axis == 1 && ysub_built(p)) {
+    if (choice == -3) {
+      p->yfuse = false;
+      return ACCFFT_AMD_OK;
+    }
+    if (choice == -1) p->yfuse = true;
+  }
   if (axis < 0 || axis > 2 || choice < -2) return ACCFFT_AMD_ERR_INVALID;
   const int n = p->n[axis];
   const bool has = p->fp == ACCFFT_AMD_F32 ? p->tf[axis].tw_sub != nullptr

@@ -1207,6 +1207,10 @@ __global__ void __launch_bounds__((S4FCfg<T, N>::TPB), (MODE == M_DERIV ? S4FCfg
 //  sum_n1 x[n2 + M n1] e^{-2 pi i n1 k1/R}.)  Compared with a z pass and a
 // y pass over N1-point columns, the traffic is the same and the y pass
 // reads and writes rows in wide segments.
+// IL (interleaved rows): the outputs go to rows k1 + R n2 instead of
+// k1 M + n2, so that the M-point stage over the rows k1 + R n2 (n2 < M)
+// writes its outputs k1 + R k2 over the same rows, in place (the 1 x 1
+// plans, whose y transform ends in the output array itself).
 // a: in / out, idist / odist (row pitches of in and out), Din / Dout (plane
 // strides), C = M, nlines = planes * M blocks, tw = the z length's static
 // table; twy: the y length's exp(-2 pi i m / N1).  The tile (64 KiB at
@@ -1214,7 +1218,7 @@ __global__ void __launch_bounds__((S4FCfg<T, N>::TPB), (MODE == M_DERIV ? S4FCfg
 // CU only unpadded (with a padded tile and the twiddles read from global
 // memory instead, the twiddle loads held 40 more VGPRs and both kernels
 // ran 6-11 % slower).
-template <typename T, int NZ, int R>
+template <typename T, int NZ, int R, bool IL = false>
 __global__ void __launch_bounds__((R / 2 * sched_minnb<NZ, 0>()))
     zy_r2c_kernel(FftArgs<T> a, const cpx<T>* __restrict__ twy) {
   constexpr int NT = sched_minnb<NZ, 0>(), LINES = R / 2, TPB = LINES * NT;
@@ -1245,8 +1249,8 @@ __global__ void __launch_bounds__((R / 2 * sched_minnb<NZ, 0>()))
   cpx<T> w[R];
 #pragma unroll
   for (int k1 = 0; k1 < R; ++k1) w[k1] = twy[n2 * k1];
-  cpx<T>* dst = reinterpret_cast<cpx<T>*>(a.out) + x * a.Dout + n2 * a.odist;
-  const int64_t rstep = M * a.odist;
+  cpx<T>* dst = reinterpret_cast<cpx<T>*>(a.out) + x * a.Dout + n2 * (IL ? R : M) * a.odist;
+  const int64_t rstep = (IL ? 1 : M) * a.odist;
   const T h = (T)0.5;
   for (int k = t; k < NH; k += TPB) {
     cpx<T> u[R];
@@ -1263,13 +1267,13 @@ __global__ void __launch_bounds__((R / 2 * sched_minnb<NZ, 0>()))
   }
 }
 
-// The inverse: rows k1 M + n2 of the y transform's M-point stage (k1 < R)
+// The inverse: rows k1 M + n2 (IL: k1 + R n2) of the y transform's M-point stage (k1 < R)
 // -> twiddles exp(+2 pi i n2 k1 / N1) and R-point inverse DFTs over k1 ->
 // the half spectra of the rows n2 + M n1, packed in pairs into complex
 // lines (imaginary parts of DC and Nyquist dropped, as c2r does) -> NZ-point
 // inverse FFTs -> real rows.  a: in / out, idist / odist, Din / Dout, C = M,
 // nlines, tw as above.
-template <typename T, int NZ, int R>
+template <typename T, int NZ, int R, bool IL = false>
 __global__ void __launch_bounds__((R / 2 * sched_minnb<NZ, 0>()))
     zy_c2r_kernel(FftArgs<T> a, const cpx<T>* __restrict__ twy) {
   constexpr int NT = sched_minnb<NZ, 0>(), LINES = R / 2, TPB = LINES * NT;
@@ -1279,8 +1283,8 @@ __global__ void __launch_bounds__((R / 2 * sched_minnb<NZ, 0>()))
   const int t = threadIdx.x;
   const int64_t blk = xcd_tile(blockIdx.x, gridDim.x, (unsigned)a.xgroup);
   const int64_t M = a.C, x = blk / M, n2 = blk % M;
-  const cpx<T>* src = reinterpret_cast<const cpx<T>*>(a.in) + x * a.Din + n2 * a.idist;
-  const int64_t rstep = M * a.idist;
+  const cpx<T>* src = reinterpret_cast<const cpx<T>*>(a.in) + x * a.Din + n2 * (IL ? R : M) * a.idist;
+  const int64_t rstep = (IL ? 1 : M) * a.idist;
   cpx<T> w[R];
 #pragma unroll
   for (int k1 = 0; k1 < R; ++k1) w[k1] = twy[n2 * k1];

@@ -8,6 +8,20 @@
 
 namespace afft {
 
+// Measurement overrides (tile widths, two-level variants, XCD chunks, pass
+// order) are read from the environment only in builds made with
+// ACCFFT_AMD_SWEEP defined (ACCFFT_AMD_SWEEP=1 python -m accfft_amd.build);
+// the shipped library ignores the environment and always uses the built-in
+// rules and the plan's MEASURE choices.
+inline const char* sweep_env(const char* name) {
+#ifdef ACCFFT_AMD_SWEEP
+  return getenv(name);
+#else
+  (void)name;
+  return nullptr;
+#endif
+}
+
 // Twiddle tables for one transform length, resident in device memory.
 template <typename T>
 struct AxisTab {
@@ -120,7 +134,7 @@ inline void dyn_lds_attrs() {
 constexpr size_t narrow_max_bytes = (size_t)256 << 20;
 inline bool narrow_tiles(int n) {
   static const unsigned mask = [] {
-    const char* e = getenv("ACCFFT_AMD_SNARROW");
+    const char* e = sweep_env("ACCFFT_AMD_SNARROW");
     return e ? (unsigned)strtoul(e, nullptr, 0) : (1u << 7) | (1u << 8);
   }();
   if (n != 128 && n != 256 && n != 512 && n != 1024 && n != 2048) return false;
@@ -133,7 +147,7 @@ inline bool narrow_tiles(int n) {
 // segment) for the lengths whose bit log2(n) is set in ACCFFT_AMD_SWIDE.
 inline bool wide_tiles(int n) {
   static const unsigned mask = [] {
-    const char* e = getenv("ACCFFT_AMD_SWIDE");
+    const char* e = sweep_env("ACCFFT_AMD_SWIDE");
     return e ? (unsigned)strtoul(e, nullptr, 0) : 0u;
   }();
   if (n != 1024 && n != 2048 && n != 4096) return false;
@@ -148,7 +162,7 @@ inline bool wide_tiles(int n) {
 // them) overrides that choice and ACCFFT_AMD_S4V ("n:v,n:v", e.g.
 // "2048:2,1024:3") picks the configuration variant of a length.
 inline unsigned env_mask(const char* name, unsigned dflt) {
-  const char* e = getenv(name);
+  const char* e = sweep_env(name);
   return e ? (unsigned)strtoul(e, nullptr, 0) : dflt;
 }
 inline int ilog2i(int n) {
@@ -163,7 +177,7 @@ inline bool four_step_default(int n) {
   return (mask >> ilog2i(n)) & 1u;
 }
 inline bool four_step_env_set() {
-  static const bool set = getenv("ACCFFT_AMD_S4") != nullptr;
+  static const bool set = sweep_env("ACCFFT_AMD_S4") != nullptr;
   return set;
 }
 // Two-level tiles for this axis: ACCFFT_AMD_S4 if set, else the tuned
@@ -178,10 +192,11 @@ inline bool four_step(const AxisTab<T>& tab) {
 // MEASURE choice (-1 none).  ACCFFT_AMD_S4V overrides; the built-in rule is
 // the persistent 2048 tiles for plain rows (measured 8 % faster on the 2048
 // x passes of the 256 x 2048 x 2048 slab) and variant 0 otherwise.
+inline int four_step_rule(int n, bool blk) { return (n == 2048 && !blk) ? 2 : 0; }
 inline int four_step_variant(int n, bool blk, int tuned = -1) {
   static const std::vector<std::pair<int, int>> tab = [] {
     std::vector<std::pair<int, int>> t;
-    const char* e = getenv("ACCFFT_AMD_S4V");
+    const char* e = sweep_env("ACCFFT_AMD_S4V");
     while (e && *e) {
       char* end = nullptr;
       const long len = strtol(e, &end, 10);
@@ -197,7 +212,7 @@ inline int four_step_variant(int n, bool blk, int tuned = -1) {
   for (const auto& kv : tab)
     if (kv.first == n) return kv.second;
   if (tuned >= 0) return tuned;
-  return (n == 2048 && !blk) ? 2 : 0;
+  return four_step_rule(n, blk);
 }
 
 // Whether S4Cfg<T, n, v> exists (the candidates of the MEASURE tuning).
@@ -211,6 +226,7 @@ inline bool s4_config_ok(int n, int v) {
       case 3: return S4Cfg<T, NN, 3>::OK;                                    \
       default: return false;                                                 \
     }
+  AFFT_S4OK(256)
   AFFT_S4OK(512)
   AFFT_S4OK(1024)
   AFFT_S4OK(2048)
@@ -265,7 +281,10 @@ hipError_t launch_s4v(int sign, FftArgs<T>& a, const AxisTab<T>& tab, hipStream_
 
 template <typename T, int N>
 hipError_t launch_s4(int sign, FftArgs<T>& a, const AxisTab<T>& tab, hipStream_t st) {
-  switch (four_step_variant(N, a.ishift != 0 || a.oshift != 0, tab.s4v)) {
+  const bool blk = a.ishift != 0 || a.oshift != 0;
+  int v = four_step_variant(N, blk, tab.s4v);
+  if (!s4_config_ok<T>(N, v)) v = four_step_rule(N, blk);  // an override with no such configuration
+  switch (v) {
     case 1: return launch_s4v<T, N, 1>(sign, a, tab, st);
     case 2: return launch_s4v<T, N, 2>(sign, a, tab, st);
     case 3: return launch_s4v<T, N, 3>(sign, a, tab, st);
@@ -335,6 +354,9 @@ hipError_t launch_fft(bool strided, int mode, int sign, FftArgs<T> a, const Axis
         default: break;
       }
     }
+#ifdef ACCFFT_AMD_SWEEP
+    // 128 KiB one-level tiles, one workgroup per CU (measured slower:
+    // 1024^3 y passes 2.19 -> 2.98 ms); sweep builds only
     if (sizeof(T) == 4 && strided && mode == M_C2C && wide_tiles(n)) {
       const int w = n == 1024 ? SCfg<T, 1024, 0, 128>::W
                   : n == 2048 ? SCfg<T, 2048, 0, 128>::W : SCfg<T, 4096, 0, 128>::W;
@@ -353,6 +375,7 @@ hipError_t launch_fft(bool strided, int mode, int sign, FftArgs<T> a, const Axis
       AFFT_WIDE(4096)
 #undef AFFT_WIDE
     }
+#endif
     if (strided && mode == M_C2C && n >= 512 && narrow_tiles(n)) {
       const int w = n == 512 ? SCfg<T, 512, 0, 32>::W
                   : n == 1024 ? SCfg<T, 1024, 0, 32>::W : SCfg<T, 2048, 0, 32>::W;
@@ -414,19 +437,54 @@ hipError_t launch_fft(bool strided, int mode, int sign, FftArgs<T> a, const Axis
   }
 }
 
-// Fused z pass and first radix-8 level of a 2048-point y transform
-// (zy_r2c_kernel / zy_c2r_kernel, NZ = 2048): one block per (plane, n2).
+// Fused z pass and first radix-8 level of the y transform (zy_r2c_kernel /
+// zy_c2r_kernel): one block per (plane, n2).  il: interleaved output rows
+// k1 + R n2 (1 x 1 plans) instead of k1 M + n2 (the slab share's
+// workspace).  FP32, z lengths 256 ... 2048.  (FP64 was measured slower
+// fused: 512^3 round trip 2.44 ms against 2.39 ms with separate passes,
+// 256^3 0.326 against 0.309 ms; its 512-point tile and twiddles take
+// 40 KiB of LDS per 256 threads, four workgroups per CU.)
 constexpr int ZY_R = 8;
 template <typename T>
-hipError_t launch_zy(int sign, const FftArgs<T>& a, const cpx<T>* twy, hipStream_t st) {
+inline bool zy_length(int nz) {
+  return sizeof(T) == 4 && (nz == 256 || nz == 512 || nz == 1024 || nz == 2048);
+}
+// 1 x 1 r2c plans whose z pass can carry the first radix-8 level of the y
+// transform: FP32, a z length of the fused kernels and N1 = 8 M with a
+// static M-point second level (M = 16 ... 256).  Split plans keep their
+// matrix-core z stages.
+template <typename T>
+inline bool zy_plain_ok(int kind, int split, const int* n) {
+  if (kind != 1 || split || n[1] % ZY_R) return false;
+  const int m = n[1] / ZY_R;
+  return zy_length<T>(n[2]) && m >= 16 && m <= 256 && (m & (m - 1)) == 0;
+}
+template <typename T, int NZ>
+hipError_t launch_zy_n(int sign, bool il, const FftArgs<T>& a, const cpx<T>* twy, hipStream_t st) {
+  dim3 g((unsigned)a.nlines), b(ZY_R / 2 * sched_minnb<NZ, 0>());
+  if (il) {
+    if (sign < 0) hipLaunchKernelGGL((zy_r2c_kernel<T, NZ, ZY_R, true>), g, b, 0, st, a, twy);
+    else hipLaunchKernelGGL((zy_c2r_kernel<T, NZ, ZY_R, true>), g, b, 0, st, a, twy);
+  } else {
+    if (sign < 0) hipLaunchKernelGGL((zy_r2c_kernel<T, NZ, ZY_R, false>), g, b, 0, st, a, twy);
+    else hipLaunchKernelGGL((zy_c2r_kernel<T, NZ, ZY_R, false>), g, b, 0, st, a, twy);
+  }
+  return hipGetLastError();
+}
+template <typename T>
+hipError_t launch_zy(int sign, int nz, bool il, const FftArgs<T>& a, const cpx<T>* twy,
+                     hipStream_t st) {
   if constexpr (sizeof(T) != 4) {
     return hipErrorNotSupported;
   } else {
     if (a.nlines <= 0) return hipSuccess;
-    dim3 g((unsigned)a.nlines), b(ZY_R / 2 * sched_minnb<2048, 0>());
-    if (sign < 0) hipLaunchKernelGGL((zy_r2c_kernel<T, 2048, ZY_R>), g, b, 0, st, a, twy);
-    else hipLaunchKernelGGL((zy_c2r_kernel<T, 2048, ZY_R>), g, b, 0, st, a, twy);
-    return hipGetLastError();
+    switch (nz) {
+      case 256: return launch_zy_n<T, 256>(sign, il, a, twy, st);
+      case 512: return launch_zy_n<T, 512>(sign, il, a, twy, st);
+      case 1024: return launch_zy_n<T, 1024>(sign, il, a, twy, st);
+      case 2048: return launch_zy_n<T, 2048>(sign, il, a, twy, st);
+      default: return hipErrorNotSupported;
+    }
   }
 }
 

@@ -32,14 +32,26 @@ struct accfft_amd_plan {
   int osize[3] = {0, 0, 0}, ostart[3] = {0, 0, 0};
   afft::AxisTab<float> tf[3];
   afft::AxisTab<double> td[3];
-  // slab share with 2048-point y and z axes (FP32): tables of the
-  // 256-point second level of the y transform (the first radix-8 level
-  // runs inside the z pass); n == 0 when not built
+  // tables of the second level (length N1/8) of a y transform whose first
+  // radix-8 level runs inside the z pass: the slab share with 2048-point y
+  // and z axes (FP32), and 1 x 1 r2c plans whose shape allows it
+  // (afft::zy_plain_ok); n == 0 when not built
   afft::AxisTab<float> tf_ysub;
+  afft::AxisTab<double> td_ysub;
+  // 1 x 1 r2c plans: the z pass carries the y transform's first radix-8
+  // level (set at plan creation when the tables exist; tuning choice -3 on
+  // the y axis turns it off, -1 restores it)
+  bool yfuse = false;
   std::vector<void*> dev_mem;  // table allocations
   void* ws[2] = {nullptr, nullptr};
   size_t ws_bytes[2] = {0, 0};
   hipStream_t stream = nullptr;
+  // stream of the previous call: when a call comes on another stream, that
+  // stream first waits for everything already queued on the previous one
+  // (the plan's workspaces are shared by all its calls)
+  hipStream_t last_stream = nullptr;
+  bool called = false;
+  hipEvent_t ev_switch = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_ms = 0.f;
   // timer contract (timer[2] = exchange, timer[4] = local FFT): set while a

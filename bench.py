@@ -133,6 +133,27 @@ def run_gpu(n, fp, steps, warmup, profile=True):
     copy_gbps = 2 * src.numel() * src.element_size() / (e0.elapsed_time(e1) / 5 * 1e-3) / 1e9
     del dst
 
+    # the same round trip with the z pass and the y transform's first level
+    # as separate passes (tuning choice -3 on the y axis), for comparison
+    unfused_ms = None
+    if tuning[1] != -3:
+        try:
+            plan.set_tuning(1, -3)
+            fused_now = plan.tuning()[1] == -3
+        except Exception:
+            fused_now = False
+        if fused_now:
+            for _ in range(warmup):
+                step()
+            torch.cuda.synchronize()
+            e0.record(stream)
+            for _ in range(steps):
+                step()
+            e1.record(stream)
+            torch.cuda.synchronize()
+            unfused_ms = e0.elapsed_time(e1) / steps
+            plan.set_tuning(1, -1)
+
     prof = {}
     if profile:
         plan.set_profiling(True)
@@ -143,7 +164,7 @@ def run_gpu(n, fp, steps, warmup, profile=True):
         prof = plan.profile()
         plan.set_profiling(False)
     plan.destroy()
-    return ms, err, prof, copy_gbps, tuning
+    return ms, err, prof, copy_gbps, tuning, unfused_ms
 
 
 SPLIT_FLAGS = {"bf16x2": 0x1000, "bf16x3": 0x2000, "fp16x2": 0x4000}
@@ -545,7 +566,7 @@ def main():
     import accfft_amd as af
 
     n = (args.n, args.n, args.n)
-    ms, err, prof, copy_gbps, tuning = run_gpu(n, af.F32, args.steps, args.warmup)
+    ms, err, prof, copy_gbps, tuning, unfused_ms = run_gpu(n, af.F32, args.steps, args.warmup)
     log(f"fp32 {n} round trip {ms:.4f} ms, rel err {err:.3e}; device copy {copy_gbps:.0f} GB/s")
     for k, (tot, cnt, b) in sorted(prof.items(), key=lambda kv: -kv[1][0]):
         log(f"  {k:40s} avg {tot / max(cnt, 1) * 1e3:8.1f} us  {b / (tot / max(cnt, 1)) / 1e6:8.1f} GB/s"
@@ -560,9 +581,10 @@ def main():
 
     extra = {}
     if not args.no_fp64:
-        ms64, err64, _, _, _ = run_gpu(n, af.F64, max(5, args.steps // 2), 2, profile=False)
+        ms64, err64, _, _, _, unf64 = run_gpu(n, af.F64, max(5, args.steps // 2), 2, profile=False)
         alg64, _ = alg_bytes_roundtrip(n, 8)
         extra = {"fp64_ms_per_step": round(ms64, 4), "fp64_rel_err": err64,
+                 "fp64_separate_z_y_ms": round(unf64, 4) if unf64 else None,
                  "fp64_pass_model_frac": round(alg64 / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         log(f"fp64 {n} round trip {ms64:.4f} ms, rel err {err64:.3e}")
 
@@ -601,6 +623,7 @@ def main():
             "compulsory_frac_of_hbm_peak": round(comp / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "device_copy_GBps": round(copy_gbps, 1),
             "tile_tuning_xyz": tuning,
+            "separate_z_y_passes_ms": round(unfused_ms, 4) if unfused_ms else None,
             "kernels": {k: {"avg_us": round(v[0] / max(v[1], 1) * 1e3, 2),
                             "alg_GBps": round(v[2] / (v[0] / max(v[1], 1) * 1e-3) / 1e9, 1)}
                         for k, v in prof.items()},

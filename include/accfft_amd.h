@@ -141,7 +141,12 @@ int accfft_amd_plan_info(const accfft_amd_plan* plan, int n[3], int isize[3],
  * in the reference; what its MEASURE planning would record): -1 the
  * built-in rule, -2 one-level tiles, v >= 0 two-level variant v.  get:
  * what ACCFFT_MEASURE chose; set: force one (ERR_INVALID when the axis
- * length has no such variant), e.g. to test each configuration. */
+ * length has no such variant), e.g. to test each configuration.
+ * On an FP32 r2c plan whose z pass carries the first radix-8 level of the y
+ * transform (N1 = 8 M, M = 16 ... 256, N2 = 256 ... 2048, not split),
+ * get reports -3 on the y axis when that fusion is off; set(1, -3) turns
+ * it off (separate z and N1-point y passes, tuned as above) and
+ * set(1, -1) restores it. */
 int accfft_amd_plan_get_tuning(const accfft_amd_plan* plan, int choice[3]);
 int accfft_amd_plan_set_tuning(accfft_amd_plan* plan, int axis, int choice);
 /* Default data pointers recorded at plan time (src/accfft_gpu.cpp:447-450). */

@@ -42,3 +42,15 @@ def rel_l2(a, b) -> float:
 
 def rng(seed: int):
     return np.random.default_rng(seed)
+
+
+def pytest_runtest_logstart(nodeid, location):
+    """GPU runs print a marker before and after each test on stdout, so that
+    the tail of a run that ends abruptly names the test that was running."""
+    if os.environ.get("ACCFFT_AMD_TEST_MARKERS"):
+        print(f"\n[START] {nodeid}", flush=True)
+
+
+def pytest_runtest_logreport(report):
+    if os.environ.get("ACCFFT_AMD_TEST_MARKERS") and report.when == "call":
+        print(f"[END] {report.nodeid} {report.outcome}", flush=True)

@@ -153,3 +153,61 @@ def test_c2c_inplace(dev, orc, shape, prec):
     assert rel_l2(buf.cpu().numpy(), orc.c2c(ref, +1)) < TOL[prec]
     assert rel_l2(buf.cpu().numpy() / np.prod(n), x) < TOL[prec]
     plan.destroy()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("shape", [(3, 128, 256), (2, 512, 512), (5, 256, 1024), (2, 2048, 2048),
+                                   (7, 1024, 512), (1, 2048, 256)])
+def test_fused_z_y_first_level(dev, orc, shape, prec):
+    """FP32 plans whose z pass carries the first radix-8 level of the y
+    transform (N1 = 8 M, M = 16 ... 256, z lengths 256 ... 2048; FP64 plans
+    keep separate passes, which are faster there): r2c,
+    c2r (also with PRESERVE_INPUT), laplace and the slow grad against the
+    oracle, and against the same plan with the fusion switched off
+    (tuning choice -3 on the y axis)."""
+    import torch
+    import accfft_amd as af
+    n = list(shape)
+    nh = n[2] // 2 + 1
+    print(f"fused z+y shape={shape} prec={prec}", flush=True)
+    fp = af.F64 if prec == "f64" else af.F32
+    cdt = torch.complex128 if prec == "f64" else torch.complex64
+    x = rng(31).standard_normal(shape)
+    xd = _t(dev, x, prec)
+    h = torch.empty((n[0], n[1], nh), dtype=cdt, device=dev)
+    plan = af.Plan("r2c", n, xd, h, fp, af.ACCFFT_ESTIMATE | af.PRESERVE_INPUT)
+    fused = prec == "f32"
+    assert plan.tuning()[1] != -3
+    plan.execute_r2c(xd, h)
+    torch.cuda.synchronize()
+    ref = orc.r2c(x)
+    assert rel_l2(h.cpu().numpy(), ref) < TOL[prec]
+    hd = _tc(dev, ref, prec)
+    back = torch.empty_like(xd)
+    plan.execute_c2r(hd, back)  # PRESERVE_INPUT: hd is kept
+    torch.cuda.synchronize()
+    refb = orc.c2r(ref, n)
+    assert rel_l2(back.cpu().numpy(), refb) < TOL[prec]
+    assert rel_l2(hd.cpu().numpy(), ref) < TOL[prec]
+    lap = torch.empty_like(xd)
+    plan.laplace(xd, lap)
+    gy = torch.empty_like(xd)
+    plan.grad(xd, None, gy, None, mask=2, slow=True)
+    torch.cuda.synchronize()
+    assert rel_l2(lap.cpu().numpy(), orc.laplace(x)) < TOL[prec]
+    assert rel_l2(gy.cpu().numpy(), orc.grad(x, 2, slow=True)[1]) < TOL[prec]
+    if not fused:  # no fused z pass at this length: nothing to switch off
+        with pytest.raises(af.AccfftError):
+            plan.set_tuning(1, -3)
+        plan.destroy()
+        return
+    # the separate passes give the same transform
+    plan.set_tuning(1, -3)
+    assert plan.tuning()[1] == -3
+    h2 = torch.empty_like(h)
+    plan.execute_r2c(xd, h2)
+    torch.cuda.synchronize()
+    assert rel_l2(h2.cpu().numpy(), ref) < TOL[prec]
+    plan.set_tuning(1, -1)
+    assert plan.tuning()[1] != -3
+    plan.destroy()

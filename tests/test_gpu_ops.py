@@ -258,3 +258,28 @@ def test_fused_two_level_operator_passes(dev, orc, shape, prec):
         gs = [x.cpu().numpy().astype(np.float64) for x in g]
         assert rel_l2(out.cpu().numpy(), orc.divergence(*gs, slow)) < tol, ("div", slow)
     plan.destroy()
+
+
+def test_plan_on_two_streams(dev, orc):
+    """One plan called from two streams: its shared workspace is ordered by
+    the library (a call on a new stream waits for the previous stream's
+    queued work), so both results match the oracle."""
+    import torch
+    import accfft_amd as af
+    n = (64, 48, 40)
+    u = [rng(200 + i).standard_normal(n) for i in range(2)]
+    ud = [torch.tensor(a, device=dev) for a in u]
+    out = [torch.empty_like(ud[0]) for _ in range(2)]
+    plan = af.Plan("r2c", n, ud[0], None, af.F64)
+    s = [torch.cuda.Stream(), torch.cuda.Stream()]
+    torch.cuda.synchronize()
+    for rep in range(3):
+        for i in range(2):
+            with torch.cuda.stream(s[i]):
+                plan.laplace(ud[i], out[i])
+                plan.inv_laplace(out[i], out[i])
+        torch.cuda.synchronize()
+        for i in range(2):
+            ref = orc.inv_laplace(orc.laplace(u[i]))
+            assert rel_l2(out[i].cpu().numpy(), ref) < 1e-12, (rep, i)
+    plan.destroy()
