@@ -184,7 +184,7 @@ def split_inputs(n, dist):
     return x, z
 
 
-def run_split(n, steps):
+def run_split(n, steps, dists=SPLIT_DISTS):
     """512^3 FP32 forward c2c and r2c: VALU butterflies vs split-precision
     matrix-core radix-16 stages, timed on the N(0,1) input.  Every form's
     output on each of the workload's three inputs is kept on the GPU so that
@@ -210,7 +210,7 @@ def run_split(n, steps):
 
     forms = [("fp32_valu", 0)] + list(SPLIT_FLAGS.items())
     h = torch.empty((n[0], n[1], n[2] // 2 + 1), dtype=torch.complex64, device=dev)
-    for dist in SPLIT_DISTS:
+    for dist in dists:
         x, z = split_inputs(n, dist)
         xd = torch.from_numpy(x).to(dev)
         zd = torch.from_numpy(z).to(dev)
@@ -250,7 +250,7 @@ def check_split(n, split, outputs):
 
     oracle.build()
     dev = torch.device("cuda:0")
-    for dist in SPLIT_DISTS:
+    for dist in sorted({k[0] for k in outputs}, key=SPLIT_DISTS.index):
         x, z = split_inputs(n, dist)
         for kind in ("r2c", "c2c"):
             ref = (oracle.r2c(x.astype(np.float64)) if kind == "r2c"
@@ -552,6 +552,9 @@ def main():
                     help="also time the CPU library FFT at 1024^3")
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 side measurement")
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision leg")
+    ap.add_argument("--split-dists", default=",".join(SPLIT_DISTS),
+                    help="inputs of the split-precision leg (comma-separated subset of "
+                         + ",".join(SPLIT_DISTS) + ")")
     ap.add_argument("--no-wide", action="store_true",
                     help="skip the slab-share and 1024^3 legs")
     ap.add_argument("--no-ops", action="store_true",
@@ -591,7 +594,10 @@ def main():
     split, split_out = None, None
     if not args.no_split:
         log("split-precision matrix-core stages vs FP32 VALU (forward transforms):")
-        split, split_out = run_split(n, max(5, args.steps // 2))
+        dists = tuple(d for d in args.split_dists.split(",") if d)
+        if not dists or any(d not in SPLIT_DISTS for d in dists) or "normal" not in dists:
+            ap.error(f"--split-dists: a subset of {SPLIT_DISTS} that includes normal")
+        split, split_out = run_split(n, max(5, args.steps // 2), dists)
         extra["split_precision"] = split
     if not args.no_wide:
         log("larger shapes:")
