@@ -104,7 +104,7 @@ void free_plan(accfft_amd_plan* p) {
   if (!p) return;
   if (p->stream) (void)hipStreamSynchronize(p->stream);
   for (void* m : p->dev_mem) (void)hipFree(m);
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < accfft_amd_plan::NWS; ++i)
     if (p->ws[i]) (void)hipFree(p->ws[i]);
   if (p->ev0) (void)hipEventDestroy(p->ev0);
   if (p->ev1) (void)hipEventDestroy(p->ev1);
@@ -133,13 +133,10 @@ int make_plan(int kind, const int n[3], void* data, void* data_out, int fp, unsi
   *out = nullptr;
   if (bad_grid(n, 1, 1, 0, 0)) return ACCFFT_AMD_ERR_INVALID;
   if (fp != ACCFFT_AMD_F32 && fp != ACCFFT_AMD_F64) return ACCFFT_AMD_ERR_INVALID;
-  // every length must fit one LDS tile of the dynamic engine
-  for (int i = 0; i < 3; ++i) {
-    const size_t csz = fp == ACCFFT_AMD_F32 ? 8 : 16;
-    const int nn = n[i];
-    if (!afft::static_length(nn) && afft::dyn_z_lds(nn, afft::dyn_lines(nn), csz) > 160 * 1024)
-      return ACCFFT_AMD_ERR_UNSUPPORTED;
-  }
+  // slab shares: every length must fit one LDS tile (lines longer than a
+  // tile run as global radix passes, fft_long.hpp, on 1 x 1 plans only)
+  for (int i = 0; i < 3 && kind == 3; ++i)
+    if (afft::long_length(n[i], fp == ACCFFT_AMD_F32 ? 8 : 16)) return ACCFFT_AMD_ERR_UNSUPPORTED;
   accfft_amd_plan* p = new (std::nothrow) accfft_amd_plan();
   if (!p) return ACCFFT_AMD_ERR_ALLOC;
   for (int i = 0; i < 3; ++i) p->n[i] = n[i];
@@ -637,7 +634,7 @@ int accfft_amd_adjoint_divergence(accfft_amd_plan* p, void* ox, void* oy, void* 
 int accfft_amd_plan_release_workspace(accfft_amd_plan* p) {
   if (!p) return ACCFFT_AMD_ERR_NULL;
   if (hipStreamSynchronize(p->stream) != hipSuccess) return ACCFFT_AMD_ERR_HIP;
-  for (int i = 0; i < 2; ++i) {
+  for (int i = 0; i < accfft_amd_plan::NWS; ++i) {
     if (p->ws[i]) (void)hipFree(p->ws[i]);
     p->ws[i] = nullptr;
     p->ws_bytes[i] = 0;
@@ -647,7 +644,9 @@ int accfft_amd_plan_release_workspace(accfft_amd_plan* p) {
 
 int accfft_amd_plan_workspace_bytes(const accfft_amd_plan* p, int64_t* bytes) {
   if (!p || !bytes) return ACCFFT_AMD_ERR_NULL;
-  *bytes = (int64_t)(p->ws_bytes[0] + p->ws_bytes[1]);
+  int64_t total = 0;
+  for (int i = 0; i < accfft_amd_plan::NWS; ++i) total += (int64_t)p->ws_bytes[i];
+  *bytes = total;
   return ACCFFT_AMD_OK;
 }
 

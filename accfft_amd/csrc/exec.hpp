@@ -105,8 +105,13 @@ inline int prof_launch(accfft_amd_plan* p, const char* name, double bytes, F&& l
 // "<z|s>fft_<mode>_n<N>_<tag>" and records its algorithmic bytes
 // (elements read + elements written, each once).
 template <typename T>
+inline int long_pass(accfft_amd_plan* p, bool strided, int mode, int sign, const FftArgs<T>& a,
+                     const AxisTab<T>& tab);
+
+template <typename T>
 inline int run_pass(accfft_amd_plan* p, const char* tag, bool strided, int mode, int sign,
                     FftArgs<T> a, const AxisTab<T>& tab, int order = -1) {
+  if (tab.longl) return long_pass<T>(p, strided, mode, sign, a, tab);
   if (order >= 0) a.reverse = reverse_pass(order) ? 1 : 0;
   if (strided) {
     const bool real_in = (mode == M_R2C || mode == M_DERIV);
@@ -162,6 +167,64 @@ inline int prof_launch(accfft_amd_plan* p, const char* name, double bytes, F&& l
   p->prof_pending.push_back({kind, e0, e1});
   if (p->prof_pending.size() > 4096) return st ? st : prof_collect(p);
   return st;
+}
+
+// A pass over lines longer than one LDS tile (tab.longl): complex lines of
+// any axis, and the real-to-complex / complex-to-real z passes (two real
+// lines per complex FFT), as global radix passes (fft_long.hpp) through
+// workspaces 2 and 3.  The fused modes, strided real passes, exchange-block
+// maps and the adjoint weights have no long-line form.
+template <typename T>
+inline int long_pass(accfft_amd_plan* p, bool strided, int mode, int sign, const FftArgs<T>& a,
+                     const AxisTab<T>& tab) {
+  using C = cpx<T>;
+  const int N = tab.n;
+  if (a.ishift || a.oshift || a.adjw || a.nlines <= 0)
+    return a.nlines <= 0 ? ACCFFT_AMD_OK : ACCFFT_AMD_ERR_UNSUPPORTED;
+  int st;
+  if (mode == M_C2C) {
+    GGeom gs, gd;
+    if (strided) {
+      gs = {a.C, a.Din, a.Sin};
+      gd = {a.C, a.Dout, a.Sout};
+    } else {
+      gs = {1, a.idist, 1};
+      gd = {1, a.odist, 1};
+    }
+    const int64_t L = a.nlines - 1;
+    const int64_t extent = (L / gd.C) * gd.D + L % gd.C + (int64_t)(N - 1) * gd.S + 1;
+    if ((st = ensure_ws(p, 2, (size_t)extent * sizeof(C)))) return st;
+    return status_of(long_c2c<T>((const C*)a.in, gs, (C*)a.out, gd, (C*)p->ws[2], a.nlines,
+                                 tab.sc, tab.tw_full, sign, strided ? 1 : 0, p->stream));
+  }
+  if (strided || (mode != M_R2C && mode != M_C2R)) return ACCFFT_AMD_ERR_UNSUPPORTED;
+  // z passes: line pairs packed into complex lines of length N (pitch N)
+  const int64_t np = (a.nlines + 1) / 2;
+  const size_t wb = (size_t)np * N * sizeof(C);
+  if ((st = ensure_ws(p, 2, wb)) || (st = ensure_ws(p, 3, wb))) return st;
+  C* z = (C*)p->ws[3];
+  const GGeom gz = {1, N, 1};
+  const dim3 b(GTPB);
+  if (mode == M_R2C) {
+    hipLaunchKernelGGL((gpack_r2c_kernel<T>), dim3(gblocks(np * N)), b, 0, p->stream,
+                       (const T*)a.in, a.idist, z, a.nlines, N);
+    if ((st = status_of(hipGetLastError()))) return st;
+    if ((st = status_of(long_c2c<T>(z, gz, z, gz, (C*)p->ws[2], np, tab.sc, tab.tw_full, -1, 0,
+                                    p->stream))))
+      return st;
+    hipLaunchKernelGGL((gsplit_r2c_kernel<T>), dim3(gblocks(np * (N / 2 + 1))), b, 0, p->stream,
+                       z, (C*)a.out, a.odist, a.nlines, N);
+    return status_of(hipGetLastError());
+  }
+  hipLaunchKernelGGL((gpack_c2r_kernel<T>), dim3(gblocks(np * (N / 2 + 1))), b, 0, p->stream,
+                     (const C*)a.in, a.idist, z, a.nlines, N);
+  if ((st = status_of(hipGetLastError()))) return st;
+  if ((st = status_of(long_c2c<T>(z, gz, z, gz, (C*)p->ws[2], np, tab.sc, tab.tw_full, +1, 0,
+                                  p->stream))))
+    return st;
+  hipLaunchKernelGGL((gunpack_c2r_kernel<T>), dim3(gblocks(np * N)), b, 0, p->stream, z,
+                     (T*)a.out, a.odist, a.nlines, N);
+  return status_of(hipGetLastError());
 }
 
 template <typename T>
@@ -225,9 +288,22 @@ struct Ops {
     return run_pass<T>(p, "x", true, M_C2C, sign, x_args(p, in, out, L), tab(p, 0),
                        sign < 0 ? 2 : 3);
   }
+  // forward FFT along x (axis 0) or y (1), symbol, inverse FFT as three
+  // passes: the fused pass for axes longer than one LDS tile
+  static int mult_unfused(accfft_amd_plan* p, int axis, const void* in, void* out, int64_t L,
+                          int sym, double scale) {
+    int st;
+    if ((st = axis == 0 ? x_c2c(p, in, out, L, -1) : y_c2c(p, in, out, L, -1))) return st;
+    const long N[3] = {p->n[0], p->n[1], p->n[2]}, os[3] = {p->n[0], p->n[1], (long)L},
+               ost[3] = {0, 0, 0};
+    if ((st = status_of(launch_mult<T>((const C*)out, (C*)out, sym, N, os, ost, scale, p->stream))))
+      return st;
+    return axis == 0 ? x_c2c(p, out, out, L, +1) : y_c2c(p, out, out, L, +1);
+  }
   // forward x, symbol, inverse x, fused (rows along x)
   static int x_mult(accfft_amd_plan* p, const void* in, void* out, int64_t L, int sym,
                     double scale) {
+    if (tab(p, 0).longl) return mult_unfused(p, 0, in, out, L, sym, scale);
     auto a = x_args(p, in, out, L);
     a.sym = sym;
     a.scale = scale;
@@ -240,6 +316,7 @@ struct Ops {
   }
   static int y_mult(accfft_amd_plan* p, const void* in, void* out, int64_t L, int sym,
                     double scale) {
+    if (tab(p, 1).longl) return mult_unfused(p, 1, in, out, L, sym, scale);
     auto a = y_args(p, in, out, L);
     a.sym = sym;
     a.scale = scale;
@@ -251,8 +328,41 @@ struct Ops {
     return run_pass<T>(p, "y", true, M_MULT, -1, a, tab(p, 1));
   }
   // fused per-axis derivative of a real field (operators_gpu.txx:172-278)
+  // The per-axis derivative along an axis longer than one LDS tile, which
+  // has no fused form: z r2c, the axis's multiplier i w(k) (along z) or a
+  // forward FFT, multiplier and inverse FFT along x / y, z c2r; out (+)=.
+  // Along x / y this is the reference's per-axis r2c -> multiplier -> c2r
+  // (the z transform pair cancels) when the axis length is even; for odd
+  // lengths the rule that zeroes k = N/2 (integer division) keeps its
+  // negative partner in a full spectrum but not in the per-axis half
+  // spectrum, so odd long x / y axes are refused.
+  static int long_deriv(accfft_amd_plan* p, int axis, const void* in, void* out, int accumulate,
+                        double sign) {
+    const int64_t N0 = p->n[0], N1 = p->n[1], N2 = p->n[2], Nh = N2 / 2 + 1;
+    if (axis != 2 && (p->n[axis] & 1)) return ACCFFT_AMD_ERR_UNSUPPORTED;
+    int st;
+    if ((st = ensure_ws(p, 0, (size_t)N0 * N1 * Nh * sizeof(C)))) return st;
+    if (accumulate && (st = ensure_ws(p, 1, (size_t)N0 * N1 * N2 * sizeof(T)))) return st;
+    void* w = p->ws[0];
+    if ((st = z_r2c(p, in, w, false))) return st;
+    if (axis == 2) {
+      const long N[3] = {p->n[0], p->n[1], p->n[2]}, os[3] = {p->n[0], p->n[1], (long)Nh},
+                 ost[3] = {0, 0, 0};
+      if ((st = status_of(launch_mult<T>((const C*)w, (C*)w, SYM_GRAD_Z, N, os, ost,
+                                         sign / (double)N2, p->stream))))
+        return st;
+    } else {
+      const double scale = sign / ((double)p->n[axis] * N2);
+      if ((st = mult_unfused(p, axis, w, w, Nh, SYM_GRAD_X + axis, scale))) return st;
+    }
+    void* dst = accumulate ? p->ws[1] : out;
+    if ((st = z_c2r(p, w, dst, false))) return st;
+    if (!accumulate) return ACCFFT_AMD_OK;
+    return status_of(launch_axpy<T>(N0 * N1 * N2, (T)1, (const T*)dst, (T*)out, p->stream));
+  }
   static int deriv(accfft_amd_plan* p, int axis, const void* in, void* out, int accumulate,
                    double sign = 1.0) {
+    if (tab(p, axis).longl) return long_deriv(p, axis, in, out, accumulate, sign);
     const int64_t N0 = p->n[0], N1 = p->n[1], N2 = p->n[2];
     FftArgs<T> a;
     bool strided = true;
@@ -709,9 +819,9 @@ struct Ops {
   // of the two-level tiles (512^3 y passes 285/315 us) it picked the
   // one-level tiles (236/240 us) on its own.  If the buffer cannot be allocated the
   // rule stays.  As with FFTW's MEASURE, the choice can change results in
-  // the last bits.
+  // the last bits.  Split-precision plans time their own (matrix-core)
+  // tiles the same way.
   static int tune(accfft_amd_plan* p) {
-    if (p->split) return ACCFFT_AMD_OK;  // split plans keep the built-in tiles
     const int64_t N0 = p->n[0], N1 = p->n[1], N2 = p->n[2], Nh = N2 / 2 + 1;
     struct Pass {
       int ax, order;

@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "fft_kernels.hpp"
+#include "fft_long.hpp"
 
 namespace afft {
 
@@ -38,6 +39,8 @@ struct AxisTab {
   int s4v = -1;
   // tiles per XCD chunk picked by the tuning (xcd_tile's g); -1 the rule
   int xg = -1;
+  // lines longer than one LDS tile: global radix passes (fft_long.hpp)
+  bool longl = false;
 };
 
 // Lengths with a matrix-core (split-precision) schedule.

@@ -116,8 +116,12 @@ def test_errors_without_gpu(af):
     # bad precision
     st = L.accfft_amd_plan_dft_3d_c2c(i3(4, 4, 4), None, None, 7, 0, ctypes.byref(h))
     assert st == 1
-    # a prime length beyond one LDS tile is unsupported (FP64: > 5120)
-    st = L.accfft_amd_plan_dft_3d_r2c(i3(8, 8, 10007), None, None, af.F64, 0, ctypes.byref(h))
+    # a slab share whose lines exceed one LDS tile is unsupported (FP64: > 5120;
+    # 1 x 1 plans run such lines as global radix passes)
+    L.accfft_amd_plan_dft_3d_r2c_slab.argtypes = [i3, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                                  ctypes.c_int, ctypes.c_uint, ctypes.c_void_p]
+    st = L.accfft_amd_plan_dft_3d_r2c_slab(i3(8, 8, 10007), 2, None, None, af.F64, 0,
+                                           ctypes.byref(h))
     assert st == 3
     # NULL plan
     assert L.accfft_amd_execute_r2c(None, None, None, None, 7) == 7
