@@ -55,6 +55,15 @@ class AccfftError(RuntimeError):
 EXCHANGE_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p)
 
+# int (*)(int comm, const void* send, const int64_t* send_bytes,
+#         const int64_t* send_offsets, void* recv, const int64_t* recv_bytes,
+#         const int64_t* recv_offsets, int nprocs, int direction, void* stream,
+#         void* user)
+_I64P = ctypes.POINTER(ctypes.c_int64)
+EXCHANGE_V_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_int, ctypes.c_void_p, _I64P, _I64P,
+                                 ctypes.c_void_p, _I64P, _I64P, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_void_p, ctypes.c_void_p)
+
 
 def lib() -> ctypes.CDLL:
     """Loads libaccfft_amd.so; raises if it has not been built."""
@@ -76,6 +85,9 @@ def lib() -> ctypes.CDLL:
     L.accfft_amd_plan_dft_3d_r2c_slab.argtypes = [_i3, ctypes.c_int, _vp, _vp, ctypes.c_int,
                                                   ctypes.c_uint, ctypes.POINTER(_vp)]
     L.accfft_amd_plan_set_exchange.argtypes = [_vp, EXCHANGE_FN, _vp]
+    L.accfft_amd_plan_dft_3d_r2c_pencil.argtypes = [_i3] + [ctypes.c_int] * 4 + [
+        _vp, _vp, ctypes.c_int, ctypes.c_uint, ctypes.POINTER(_vp)]
+    L.accfft_amd_plan_set_exchange_v.argtypes = [_vp, EXCHANGE_V_FN, _vp]
     L.accfft_amd_destroy_plan.argtypes = [_vp]
     L.accfft_amd_plan_set_stream.argtypes = [_vp, _vp]
     L.accfft_amd_plan_sync.argtypes = [_vp]
@@ -225,18 +237,28 @@ class Plan:
     same buffer, which includes passing neither."""
 
     def __init__(self, kind: str, n: Sequence[int], data=None, data_out=None,
-                 precision: int = F64, flags: int = ACCFFT_MEASURE, nprocs: int = 1):
-        """``kind`` is "r2c", "c2c" or "r2c_slab" (one rank's share of an r2c
-        on an ``nprocs`` x 1 grid, ``n`` global; see include/accfft_amd.h)."""
+                 precision: int = F64, flags: int = ACCFFT_MEASURE, nprocs: int = 1,
+                 grid=(1, 1), coords=(0, 0)):
+        """``kind`` is "r2c", "c2c", "r2c_slab" (one rank's share of an r2c
+        on an ``nprocs`` x 1 grid, ``n`` global) or "r2c_pencil" (the share
+        of the rank at ``coords`` of a ``grid`` = (p0, p1) process grid,
+        the reference's accfft_plan_dft_3d_r2c_gpu on c_dims = grid); see
+        include/accfft_amd.h."""
         self.kind = kind
         self.n = tuple(int(v) for v in n)
         self.precision = precision
         self.nprocs = nprocs
+        self.grid, self.coords = tuple(grid), tuple(coords)
         h = _vp()
         if kind == "r2c_slab":
             _check(lib().accfft_amd_plan_dft_3d_r2c_slab(_i3(*self.n), nprocs, _ptr(data),
                                                          _ptr(data_out), precision, flags,
                                                          ctypes.byref(h)), "plan_dft_3d_r2c_slab")
+        elif kind == "r2c_pencil":
+            _check(lib().accfft_amd_plan_dft_3d_r2c_pencil(
+                _i3(*self.n), self.grid[0], self.grid[1], self.coords[0], self.coords[1],
+                _ptr(data), _ptr(data_out), precision, flags, ctypes.byref(h)),
+                "plan_dft_3d_r2c_pencil")
         else:
             fn = (lib().accfft_amd_plan_dft_3d_r2c if kind == "r2c"
                   else lib().accfft_amd_plan_dft_3d_c2c)
@@ -416,6 +438,27 @@ class Plan:
                     return 5
             cb = EXCHANGE_FN(tramp)
         _check(lib().accfft_amd_plan_set_exchange(self._h, cb, None), "set_exchange")
+        self._exchange = cb  # keep the thunk alive while the plan holds it
+
+    def set_exchange_v(self, fn):
+        """Exchange hook of a pencil plan: ``fn(comm, send, send_bytes,
+        send_offsets, recv, recv_bytes, recv_offsets, direction, stream)``
+        (pointers and the stream as ints, the per-peer counts and offsets as
+        lists of ints; comm 0 = the row communicator of the p1 ranks sharing
+        this rank's c0, 1 = the column communicator of the p0 ranks sharing
+        its c1) returning a status; None = identity."""
+        if fn is None:
+            cb = EXCHANGE_V_FN()
+        else:
+            def tramp(comm, send, sb, so, recv, rb, ro, nprocs, direction, stream, user):
+                try:
+                    lst = lambda a: [a[i] for i in range(nprocs)]
+                    return int(fn(comm, send, lst(sb), lst(so), recv, lst(rb), lst(ro),
+                                  direction, stream))
+                except Exception:  # a Python error must not unwind through C
+                    return 5
+            cb = EXCHANGE_V_FN(tramp)
+        _check(lib().accfft_amd_plan_set_exchange_v(self._h, cb, None), "set_exchange_v")
         self._exchange = cb  # keep the thunk alive while the plan holds it
 
     # adjoints of the real-linear maps above (no counterpart in the reference)

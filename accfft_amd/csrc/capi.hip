@@ -13,6 +13,7 @@
 #include <cmath>
 #include <mutex>
 #include <new>
+#include <vector>
 
 #include "../../include/accfft_amd.h"
 #include "dispatch.hpp"
@@ -20,26 +21,39 @@
 
 namespace {
 
+// One axis of dfft_get_local_size_t (src/accfft_common.cpp:49-91): blocks
+// of ceil(N / P); the block that would run past N is cut, later ones are
+// empty and start at N.
+void split_block(int N, int P, int c, int* size, int* start) {
+  *size = (int)std::ceil(N / (double)P);
+  *start = *size * c;
+  if ((N - *size * c) < *size) {
+    *size = N - *size * c;
+    *size *= (int)(*size > 0);
+    *start = N - *size;
+  }
+}
+
 // dfft_get_local_size_t (src/accfft_common.cpp:49-91); returns bytes.
 int64_t local_size(int N0, int N1, int tuple, int* isize, int* istart, int p0, int p1, int c0,
                    int c1, int elem) {
   isize[2] = tuple;
-  isize[0] = (int)std::ceil(N0 / (double)p0);
-  isize[1] = (int)std::ceil(N1 / (double)p1);
-  istart[0] = isize[0] * c0;
-  istart[1] = isize[1] * c1;
   istart[2] = 0;
-  if ((N0 - isize[0] * c0) < isize[0]) {
-    isize[0] = N0 - isize[0] * c0;
-    isize[0] *= (int)(isize[0] > 0);
-    istart[0] = N0 - isize[0];
-  }
-  if ((N1 - isize[1] * c1) < isize[1]) {
-    isize[1] = N1 - isize[1] * c1;
-    isize[1] *= (int)(isize[1] > 0);
-    istart[1] = N1 - isize[1];
-  }
+  split_block(N0, p0, c0, &isize[0], &istart[0]);
+  split_block(N1, p1, c1, &isize[1], &istart[1]);
   return (int64_t)isize[0] * isize[1] * isize[2] * elem;
+}
+
+// Per-peer sizes and starts of one axis's split over P ranks.
+void split_all(int N, int P, std::vector<int64_t>& n, std::vector<int64_t>& s) {
+  n.resize(P);
+  s.resize(P);
+  for (int q = 0; q < P; ++q) {
+    int a, b;
+    split_block(N, P, q, &a, &b);
+    n[q] = a;
+    s[q] = b;
+  }
 }
 
 bool bad_grid(const int n[3], int p0, int p1, int c0, int c1) {
@@ -109,8 +123,8 @@ void free_plan(accfft_amd_plan* p) {
   if (p->ev0) (void)hipEventDestroy(p->ev0);
   if (p->ev1) (void)hipEventDestroy(p->ev1);
   if (p->ev_switch) (void)hipEventDestroy(p->ev_switch);
-  if (p->evx0) (void)hipEventDestroy(p->evx0);
-  if (p->evx1) (void)hipEventDestroy(p->evx1);
+  for (hipEvent_t e : p->evx)
+    if (e) (void)hipEventDestroy(e);
   for (auto& q : p->prof_pending) {
     (void)hipEventDestroy(q.a);
     (void)hipEventDestroy(q.b);
@@ -133,9 +147,10 @@ int make_plan(int kind, const int n[3], void* data, void* data_out, int fp, unsi
   *out = nullptr;
   if (bad_grid(n, 1, 1, 0, 0)) return ACCFFT_AMD_ERR_INVALID;
   if (fp != ACCFFT_AMD_F32 && fp != ACCFFT_AMD_F64) return ACCFFT_AMD_ERR_INVALID;
-  // slab shares: every length must fit one LDS tile (lines longer than a
-  // tile run as global radix passes, fft_long.hpp, on 1 x 1 plans only)
-  for (int i = 0; i < 3 && kind == 3; ++i)
+  // slab and pencil shares: every length must fit one LDS tile (lines
+  // longer than a tile run as global radix passes, fft_long.hpp, on 1 x 1
+  // plans only)
+  for (int i = 0; i < 3 && kind >= 3; ++i)
     if (afft::long_length(n[i], fp == ACCFFT_AMD_F32 ? 8 : 16)) return ACCFFT_AMD_ERR_UNSUPPORTED;
   accfft_amd_plan* p = new (std::nothrow) accfft_amd_plan();
   if (!p) return ACCFFT_AMD_ERR_ALLOC;
@@ -163,13 +178,22 @@ int make_plan(int kind, const int n[3], void* data, void* data_out, int fp, unsi
     return ACCFFT_AMD_ERR_HIP;
   }
   int st = fp == ACCFFT_AMD_F32 ? afft_tables_f32(p) : afft_tables_f64(p);
-  if (!st && kind != 3) st = maybe_tune(p);
+  if (!st && kind < 3) st = maybe_tune(p);
   if (st) {
     free_plan(p);
     return st;
   }
   *out = p;
   return ACCFFT_AMD_OK;
+}
+
+// Whether a plan's real side / complex side holds no elements (a pencil
+// share of a ragged split can be empty; its buffers may then be NULL).
+bool real_empty(const accfft_amd_plan* p) {
+  return (int64_t)p->isize[0] * p->isize[1] * p->isize[2] == 0;
+}
+bool cplx_empty(const accfft_amd_plan* p) {
+  return (int64_t)p->osize[0] * p->osize[1] * p->osize[2] == 0;
 }
 
 // Whether the plan holds the tables of a y transform's second level.
@@ -194,7 +218,7 @@ int timed(accfft_amd_plan* p, double* timer, F&& body) {
   const bool want = timer || (p->flags & ACCFFT_AMD_BLOCKING);
   if (!want) return body();
   p->timing = timer != nullptr;
-  p->comm_recorded = false;
+  p->comm_pairs = 0;
   (void)hipEventRecord(p->ev0, p->stream);
   int st = body();
   (void)hipEventRecord(p->ev1, p->stream);
@@ -204,7 +228,11 @@ int timed(accfft_amd_plan* p, double* timer, F&& body) {
     if (hipEventSynchronize(p->ev1) != hipSuccess) return ACCFFT_AMD_ERR_HIP;
     float ms = 0.f, comm = 0.f;
     (void)hipEventElapsedTime(&ms, p->ev0, p->ev1);
-    if (p->comm_recorded) (void)hipEventElapsedTime(&comm, p->evx0, p->evx1);
+    for (int k = 0; k < p->comm_pairs; ++k) {
+      float c = 0.f;
+      (void)hipEventElapsedTime(&c, p->evx[2 * k], p->evx[2 * k + 1]);
+      comm += c;
+    }
     p->last_ms = ms;
     // doc/timing.md: timer[2] communication, timer[4] local FFT
     if (timer) {
@@ -323,6 +351,33 @@ int accfft_amd_plan_dft_3d_r2c_slab(const int n[3], int nprocs, void* data, void
   return ACCFFT_AMD_OK;
 }
 
+int accfft_amd_plan_dft_3d_r2c_pencil(const int n[3], int p0, int p1, int c0, int c1,
+                                      void* data, void* data_out, int fp, unsigned flags,
+                                      accfft_amd_plan** plan) {
+  if (!plan || !n) return ACCFFT_AMD_ERR_NULL;
+  *plan = nullptr;
+  if (bad_grid(n, p0, p1, c0, c1)) return ACCFFT_AMD_ERR_INVALID;
+  if (p0 > 64 || p1 > 64) return ACCFFT_AMD_ERR_UNSUPPORTED;
+  if (data && data == data_out) return ACCFFT_AMD_ERR_UNSUPPORTED;  // out of place only
+  accfft_amd_plan* p = nullptr;
+  int st = make_plan(4, n, data, data_out, fp, flags, &p);
+  if (st) return st;
+  p->inplace = false;  // out of place (buffers may also be given only at execute)
+  p->p0 = p0;
+  p->p1 = p1;
+  p->c0 = c0;
+  p->c1 = c1;
+  p->alloc_max = size_r2c(n, p->isize, p->istart, p->osize, p->ostart, p0, p1, c0, c1,
+                          elem_of(fp));
+  const int Nh = n[2] / 2 + 1;
+  split_all(n[0], p0, p->xn, p->xs);
+  split_all(n[1], p1, p->y1n, p->y1s);
+  split_all(n[1], p0, p->yn, p->ys);
+  split_all(Nh, p1, p->zn, p->zs);
+  *plan = p;
+  return ACCFFT_AMD_OK;
+}
+
 int accfft_amd_plan_get_tuning(const accfft_amd_plan* p, int choice[3]) {
   if (!p || !choice) return ACCFFT_AMD_ERR_NULL;
   for (int ax = 0; ax < 3; ++ax)
@@ -367,6 +422,14 @@ int accfft_amd_plan_set_exchange(accfft_amd_plan* p, accfft_amd_exchange_fn fn, 
   return ACCFFT_AMD_OK;
 }
 
+int accfft_amd_plan_set_exchange_v(accfft_amd_plan* p, accfft_amd_exchange_v_fn fn, void* user) {
+  if (!p) return ACCFFT_AMD_ERR_NULL;
+  if (p->kind != 4) return ACCFFT_AMD_ERR_NOT_BAKED;
+  p->exchange_v = fn;
+  p->exchange_user = user;
+  return ACCFFT_AMD_OK;
+}
+
 int accfft_amd_destroy_plan(accfft_amd_plan* plan) {
   if (!plan) return ACCFFT_AMD_ERR_NULL;
   free_plan(plan);
@@ -407,15 +470,23 @@ int accfft_amd_plan_data(const accfft_amd_plan* p, void** data, void** data_out)
 int accfft_amd_execute_r2c(accfft_amd_plan* p, const void* in, void* out, double* timer,
                            unsigned xyz) {
   if (!p) return ACCFFT_AMD_ERR_NULL;
-  if (p->kind != 1 && p->kind != 3) return ACCFFT_AMD_ERR_NOT_BAKED;
+  if (p->kind < 1 || p->kind == 2 || p->kind > 4) return ACCFFT_AMD_ERR_NOT_BAKED;
   if (!in) in = p->data;
   if (!out) out = p->data_out;
-  if (!in || !out) return ACCFFT_AMD_ERR_NULL;
+  if ((!in && !real_empty(p)) || (!out && !cplx_empty(p))) return ACCFFT_AMD_ERR_NULL;
   xyz &= 7u;
   if (p->kind == 3) {
     if (xyz != 7u) return ACCFFT_AMD_ERR_UNSUPPORTED;
     return timed(p, timer, [&] {
       return p->fp == ACCFFT_AMD_F32 ? afft_r2c_slab_f32(p, in, out) : afft_r2c_slab_f64(p, in, out);
+    });
+  }
+  if (p->kind == 4) {
+    if (xyz != 7u) return ACCFFT_AMD_ERR_UNSUPPORTED;
+    if (in && in == out) return ACCFFT_AMD_ERR_UNSUPPORTED;
+    return timed(p, timer, [&] {
+      return p->fp == ACCFFT_AMD_F32 ? afft_r2c_pencil_f32(p, in, out)
+                                     : afft_r2c_pencil_f64(p, in, out);
     });
   }
   return timed(p, timer, [&] {
@@ -426,15 +497,23 @@ int accfft_amd_execute_r2c(accfft_amd_plan* p, const void* in, void* out, double
 int accfft_amd_execute_c2r(accfft_amd_plan* p, void* in, void* out, double* timer,
                            unsigned xyz) {
   if (!p) return ACCFFT_AMD_ERR_NULL;
-  if (p->kind != 1 && p->kind != 3) return ACCFFT_AMD_ERR_NOT_BAKED;
+  if (p->kind < 1 || p->kind == 2 || p->kind > 4) return ACCFFT_AMD_ERR_NOT_BAKED;
   if (!in) in = p->data_out;
   if (!out) out = p->data;
-  if (!in || !out) return ACCFFT_AMD_ERR_NULL;
+  if ((!in && !cplx_empty(p)) || (!out && !real_empty(p))) return ACCFFT_AMD_ERR_NULL;
   xyz &= 7u;
   if (p->kind == 3) {
     if (xyz != 7u) return ACCFFT_AMD_ERR_UNSUPPORTED;
     return timed(p, timer, [&] {
       return p->fp == ACCFFT_AMD_F32 ? afft_c2r_slab_f32(p, in, out) : afft_c2r_slab_f64(p, in, out);
+    });
+  }
+  if (p->kind == 4) {
+    if (xyz != 7u) return ACCFFT_AMD_ERR_UNSUPPORTED;
+    if (in && in == out) return ACCFFT_AMD_ERR_UNSUPPORTED;
+    return timed(p, timer, [&] {
+      return p->fp == ACCFFT_AMD_F32 ? afft_c2r_pencil_f32(p, in, out)
+                                     : afft_c2r_pencil_f64(p, in, out);
     });
   }
   return timed(p, timer, [&] {

@@ -13,6 +13,8 @@
 #include <stdio.h>
 #include <stdlib.h>
 
+#include <algorithm>
+#include <utility>
 #include <vector>
 
 #include "plan.hpp"
@@ -491,16 +493,25 @@ struct Ops {
   // Calls the plan's exchange hook; in a timed call its span is recorded
   // for timer[2] (the reference's communication time, doc/timing.md).
   static int exchange(accfft_amd_plan* p, const void* send, void* recv, int64_t block, int dir) {
-    if (p->timing) {
-      if ((!p->evx0 && hipEventCreate(&p->evx0) != hipSuccess) ||
-          (!p->evx1 && hipEventCreate(&p->evx1) != hipSuccess))
-        return ACCFFT_AMD_ERR_HIP;
-      (void)hipEventRecord(p->evx0, p->stream);
+    return comm_span(p, [&] {
+      return p->exchange(send, recv, block, p->nprocs, dir, p->stream, p->exchange_user);
+    });
+  }
+  // Runs one exchange hook call; in a timed call its span is bracketed by
+  // the next pair of the plan's exchange events (summed into timer[2]).
+  template <typename F>
+  static int comm_span(accfft_amd_plan* p, F&& call) {
+    const int k = p->comm_pairs;
+    const bool rec = p->timing && k < accfft_amd_plan::NEVX / 2;
+    if (rec) {
+      for (int e = 2 * k; e < 2 * k + 2; ++e)
+        if (!p->evx[e] && hipEventCreate(&p->evx[e]) != hipSuccess) return ACCFFT_AMD_ERR_HIP;
+      (void)hipEventRecord(p->evx[2 * k], p->stream);
     }
-    const int st = p->exchange(send, recv, block, p->nprocs, dir, p->stream, p->exchange_user);
-    if (p->timing) {
-      (void)hipEventRecord(p->evx1, p->stream);
-      p->comm_recorded = true;
+    const int st = call();
+    if (rec) {
+      (void)hipEventRecord(p->evx[2 * k + 1], p->stream);
+      p->comm_pairs = k + 1;
     }
     return st;
   }
@@ -667,6 +678,160 @@ struct Ops {
     if (P == 1) ay.ishift = 0;
     if ((st = run_pass<T>(p, "y", true, M_C2C, +1, ay, tab(p, 1), 4))) return st;
     auto az = zargs(ybuf, out, n0 * N1, Np, N2);
+    return run_pass<T>(p, "z", false, M_C2R, +1, az, tab(p, 2), 5);
+  }
+  // ---------------------------------------------------------- pencil share
+  // One rank's share of an r2c on a p0 x p1 grid (src/accfft_gpu.cpp:471-623):
+  // [n0][n1][N2] real -> z r2c -> T_plan_1 over the p1 ranks of the row
+  // communicator -> [n0][N1][nz] -> y c2c -> T_plan_2 over the p0 ranks of
+  // the column communicator -> [N0][ny][nz] -> x c2c (n0, n1 = isize0, 1;
+  // ny, nz = osize1, 2).  Each transpose is the pack of memcpy_v1_h1
+  // (accfft_amd_pencil_pack), the exchange (the plan's exchange_v hook, an
+  // MPI_Alltoallv in the reference), and the unpack of memcpy_v1_h2
+  // (accfft_amd_pencil_unpack) or, for T_plan_2, the receive straight into
+  // the output rows; the inverse transposes pack the row ranges and
+  // scatter the received column ranges back (afft_pencil_scatter).  A
+  // communicator of one rank exchanges nothing (transpose_gpu.cpp:1814-1816).
+  struct Xchg {  // per-peer byte counts and offsets of one exchange
+    int64_t sb[64], so[64], rb[64], ro[64];
+    int P;
+    bool same() const {  // every block sent has the size of the one received
+      for (int q = 0; q < P; ++q)
+        if (sb[q] != rb[q]) return false;
+      return true;
+    }
+  };
+  // T_plan_1 (comm 0) and T_plan_2 (comm 1) of the forward transform; the
+  // inverse exchanges are the same with send and receive swapped.
+  static void pencil_tables(const accfft_amd_plan* p, Xchg& t1, Xchg& t2) {
+    const int64_t cs = sizeof(C), n0 = p->isize[0], n1 = p->isize[1];
+    const int64_t ny = p->osize[1], nz = p->osize[2];
+    t1.P = p->p1;
+    for (int q = 0; q < p->p1; ++q) {
+      t1.sb[q] = n0 * n1 * p->zn[q] * cs;  // z columns of peer q, [n0][n1][zn_q]
+      t1.so[q] = n0 * n1 * p->zs[q] * cs;
+      t1.rb[q] = n0 * p->y1n[q] * nz * cs;  // peer q's y rows, [n0][y1n_q][nz]
+      t1.ro[q] = n0 * p->y1s[q] * nz * cs;
+    }
+    t2.P = p->p0;
+    for (int q = 0; q < p->p0; ++q) {
+      t2.sb[q] = n0 * p->yn[q] * nz * cs;  // y rows of peer q, [n0][yn_q][nz]
+      t2.so[q] = n0 * p->ys[q] * nz * cs;
+      t2.rb[q] = p->xn[q] * ny * nz * cs;  // peer q's x rows, [xn_q][ny][nz]
+      t2.ro[q] = p->xs[q] * ny * nz * cs;
+    }
+  }
+  static void swap_sides(Xchg& t) {
+    for (int q = 0; q < t.P; ++q) {
+      std::swap(t.sb[q], t.rb[q]);
+      std::swap(t.so[q], t.ro[q]);
+    }
+  }
+  // Runs one exchange through the hook; returns where the received blocks
+  // are (recv, or send itself for the identity exchange of a plan without
+  // a hook).
+  static int pencil_exchange(accfft_amd_plan* p, int comm, const Xchg& t, const void* send,
+                             void* recv, int dir, const void** got) {
+    if (!p->exchange_v) {
+      *got = send;
+      return ACCFFT_AMD_OK;
+    }
+    *got = recv;
+    return comm_span(p, [&] {
+      return p->exchange_v(comm, send, t.sb, t.so, recv, t.rb, t.ro, t.P, dir, p->stream,
+                           p->exchange_user);
+    });
+  }
+  static int pencil_ws(accfft_amd_plan* p) {
+    const int64_t n0 = p->isize[0], n1 = p->isize[1], N1 = p->n[1], Nh = p->n[2] / 2 + 1;
+    const int64_t nz = p->osize[2];
+    const size_t b = (size_t)std::max(n0 * n1 * Nh, n0 * N1 * nz) * sizeof(C);
+    const bool hook = p->exchange_v != nullptr;
+    int st;
+    if ((st = ensure_ws(p, 0, b))) return st;
+    if ((p->p1 > 1 || (p->p0 > 1 && hook)) && (st = ensure_ws(p, 1, b))) return st;
+    if (p->p1 > 1 && hook && (st = ensure_ws(p, 2, b))) return st;
+    return ACCFFT_AMD_OK;
+  }
+  static int r2c_pencil(accfft_amd_plan* p, const void* in, void* out) {
+    const int64_t N1 = p->n[1], N2 = p->n[2], Nh = N2 / 2 + 1;
+    const int64_t n0 = p->isize[0], n1 = p->isize[1], ny = p->osize[1], nz = p->osize[2];
+    const int P0 = p->p0, P1 = p->p1;
+    const int es = (int)sizeof(C);
+    Xchg t1, t2;
+    pencil_tables(p, t1, t2);
+    if (!p->exchange_v && ((P1 > 1 && !t1.same()) || (P0 > 1 && !t2.same())))
+      return ACCFFT_AMD_ERR_UNSUPPORTED;
+    int st;
+    if ((st = pencil_ws(p))) return st;
+    void* A = p->ws[0];
+    auto az = zargs(in, A, n0 * n1, N2, Nh);
+    if ((st = run_pass<T>(p, "z", false, M_R2C, -1, az, tab(p, 2), 0))) return st;
+    if (P1 > 1) {  // [n0][n1][Nh] -> [n0][N1][nz]
+      const void* got;
+      if ((st = accfft_amd_pencil_pack(P1, (int)n0, (int)n1, 1, p->zn.data(), p->ws[1], A,
+                                       n1 * Nh, Nh, p->zs.data(), es, p->stream)))
+        return st;
+      if ((st = pencil_exchange(p, 0, t1, p->ws[1], p->ws[2], -1, &got))) return st;
+      if ((st = accfft_amd_pencil_unpack(P1, (int)n0, p->y1s.data(), p->y1n.data(), A, N1 * nz,
+                                         nz, 1, got, es, p->stream)))
+        return st;
+    }
+    // y c2c over [n0][N1][nz]; with one rank along x it is already the output
+    void* yout = P0 > 1 ? A : out;
+    auto ay = sargs(A, yout, n0 * nz, nz, N1 * nz, N1 * nz, nz, nz);
+    if ((st = run_pass<T>(p, "y", true, M_C2C, -1, ay, tab(p, 1), 1))) return st;
+    if (P0 > 1) {  // [n0][N1][nz] -> [N0][ny][nz], received in place
+      void* send = p->exchange_v ? p->ws[1] : out;
+      const void* got;
+      if ((st = accfft_amd_pencil_pack(P0, (int)n0, 1, nz, p->yn.data(), send, A, N1 * nz, N1,
+                                       p->ys.data(), es, p->stream)))
+        return st;
+      if ((st = pencil_exchange(p, 1, t2, send, out, -1, &got))) return st;
+    }
+    auto ax = sargs(out, out, ny * nz, ny * nz, 0, 0, ny * nz, ny * nz);
+    return run_pass<T>(p, "x", true, M_C2C, -1, ax, tab(p, 0), 2);
+  }
+  // The inverse: x c2c in place on the input (which it overwrites, as the
+  // reference's c2r does), T_plan_2i, y c2c, T_plan_1i, z c2r.
+  static int c2r_pencil(accfft_amd_plan* p, void* in, void* out) {
+    const int64_t N1 = p->n[1], N2 = p->n[2], Nh = N2 / 2 + 1;
+    const int64_t n0 = p->isize[0], n1 = p->isize[1], ny = p->osize[1], nz = p->osize[2];
+    const int P0 = p->p0, P1 = p->p1;
+    const int es = (int)sizeof(C);
+    Xchg t1, t2;
+    pencil_tables(p, t1, t2);
+    swap_sides(t1);
+    swap_sides(t2);
+    if (!p->exchange_v && ((P1 > 1 && !t1.same()) || (P0 > 1 && !t2.same())))
+      return ACCFFT_AMD_ERR_UNSUPPORTED;
+    int st;
+    if ((st = pencil_ws(p))) return st;
+    void* A = p->ws[0];
+    auto ax = sargs(in, in, ny * nz, ny * nz, 0, 0, ny * nz, ny * nz);
+    if ((st = run_pass<T>(p, "x", true, M_C2C, +1, ax, tab(p, 0), 3))) return st;
+    const void* ysrc = in;
+    if (P0 > 1) {  // [N0][ny][nz] -> [n0][N1][nz]
+      const void* got;
+      if ((st = pencil_exchange(p, 1, t2, in, p->ws[1], +1, &got))) return st;
+      if ((st = afft_pencil_scatter(P0, (int)n0, 1, nz, p->yn.data(), got, A, N1 * nz, N1,
+                                    p->ys.data(), es, p->stream)))
+        return st;
+      ysrc = A;
+    }
+    auto ay = sargs(ysrc, A, n0 * nz, nz, N1 * nz, N1 * nz, nz, nz);
+    if ((st = run_pass<T>(p, "y", true, M_C2C, +1, ay, tab(p, 1), 4))) return st;
+    if (P1 > 1) {  // [n0][N1][nz] -> [n0][n1][Nh]
+      const void* got;
+      if ((st = accfft_amd_pencil_pack(P1, (int)n0, 1, nz, p->y1n.data(), p->ws[1], A, N1 * nz,
+                                       N1, p->y1s.data(), es, p->stream)))
+        return st;
+      if ((st = pencil_exchange(p, 0, t1, p->ws[1], p->ws[2], +1, &got))) return st;
+      if ((st = afft_pencil_scatter(P1, (int)n0, (int)n1, 1, p->zn.data(), got, A, n1 * Nh, Nh,
+                                    p->zs.data(), es, p->stream)))
+        return st;
+    }
+    auto az = zargs(A, out, n0 * n1, Nh, N2);
     return run_pass<T>(p, "z", false, M_C2R, +1, az, tab(p, 2), 5);
   }
   static int ilog2(int64_t v) {

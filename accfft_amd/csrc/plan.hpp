@@ -19,10 +19,20 @@
 struct accfft_amd_plan {
   int n[3] = {0, 0, 0};
   int fp = ACCFFT_AMD_F64;
-  int kind = 0;  // 1 = r2c, 2 = c2c, 3 = r2c slab share (accfft_amd_plan_dft_3d_r2c_slab)
+  // 1 = r2c, 2 = c2c, 3 = r2c slab share (accfft_amd_plan_dft_3d_r2c_slab),
+  // 4 = r2c pencil share (accfft_amd_plan_dft_3d_r2c_pencil)
+  int kind = 0;
   int nprocs = 1;  // slab share: ranks of the 1-D split (n[0] and n[1] are global)
   accfft_amd_exchange_fn exchange = nullptr;  // slab share: NULL = identity
+  accfft_amd_exchange_v_fn exchange_v = nullptr;  // pencil share: NULL = identity
   void* exchange_user = nullptr;
+  // pencil share: the p0 x p1 grid, this rank's coordinates, and the
+  // per-peer counts and starts of the ceil-block splits
+  // (dfft_get_local_size, src/accfft_common.cpp:49-91): x rows over p0
+  // (xn, xs), input y rows over p1 (y1n, y1s), output y rows over p0
+  // (yn, ys), output z columns of the half spectrum over p1 (zn, zs)
+  int p0 = 1, p1 = 1, c0 = 0, c1 = 0;
+  std::vector<int64_t> xn, xs, y1n, y1s, yn, ys, zn, zs;
   bool inplace = false;
   unsigned flags = 0;
   void* data = nullptr;
@@ -58,10 +68,12 @@ struct accfft_amd_plan {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   float last_ms = 0.f;
   // timer contract (timer[2] = exchange, timer[4] = local FFT): set while a
-  // timed call runs; the slab plan brackets its exchange hook with evx0/1
+  // timed call runs; each exchange hook call of the call is bracketed by
+  // the event pair evx[2 k], evx[2 k + 1] (k < comm_pairs)
   bool timing = false;
-  bool comm_recorded = false;
-  hipEvent_t evx0 = nullptr, evx1 = nullptr;
+  static constexpr int NEVX = 4;
+  int comm_pairs = 0;
+  hipEvent_t evx[NEVX] = {nullptr, nullptr, nullptr, nullptr};
   int device = 0;
   int split = 0;  // afft::SplitKind of the radix-16 stages (FP32 plans)
   // per-launch event profiling (accfft_amd_plan_set_profiling)

@@ -136,10 +136,11 @@ struct RankTab {
 // Pack (memcpy_v1_h1): segment (proc, h, i) copies the cnt[proc] tuples of
 // row i of batch h that start at column start[proc] into the send buffer
 // [proc][h][i][cnt[proc] tuples].  a = cnt, b = start, pre = prefix of cnt.
+// scatter = 1: the inverse copy (packed segment -> its place in the rows).
 __global__ void __launch_bounds__(TPB)
     pencil_pack_kernel(int64_t howmany, int64_t local_n0, int64_t t, int64_t N1, int64_t idist,
                        RankTab rk, const char* __restrict__ data, char* __restrict__ send,
-                       int esz) {
+                       int esz, int scatter) {
   const int64_t per = howmany * local_n0;
   const int64_t seg = blockIdx.x;
   const int proc = (int)(seg / per);
@@ -151,7 +152,8 @@ __global__ void __launch_bounds__(TPB)
   const int64_t len = cnt * t * esz;
   for (int64_t off = (int64_t)blockIdx.y * CHUNK; off < len; off += (int64_t)gridDim.y * CHUNK) {
     const int64_t l = (len - off) < CHUNK ? (len - off) : CHUNK;
-    copy_bytes(data + src * esz + off, send + dst * esz + off, l);
+    if (scatter) copy_bytes(send + dst * esz + off, const_cast<char*>(data) + src * esz + off, l);
+    else copy_bytes(data + src * esz + off, send + dst * esz + off, l);
   }
 }
 
@@ -201,7 +203,47 @@ int launch_tuples(int kind, int r, int c, int64_t t, int64_t t2, const void* in,
   return hipGetLastError() == hipSuccess ? ACCFFT_AMD_OK : ACCFFT_AMD_ERR_HIP;
 }
 
+// Column-range pack (memcpy_v1_h1) and, scatter = 1, its inverse.
+int pack_cols(int scatter, int nprocs_1, int howmany, int local_n0, int64_t n_tuples,
+              const int64_t* local_n1_proc, void* send_recv, void* data, int64_t idist,
+              int64_t N1, const int64_t* local_1_start_proc, int elem_size, void* stream) {
+  if (nprocs_1 < 1 || nprocs_1 > MAXP || howmany < 0 || local_n0 < 0 || n_tuples < 0 ||
+      elem_size <= 0 || idist < 0 || N1 < 0)
+    return ACCFFT_AMD_ERR_INVALID;
+  if (!local_n1_proc || !local_1_start_proc) return ACCFFT_AMD_ERR_NULL;
+  RankTab rk{};
+  int64_t maxlen = 0;
+  rk.pre[0] = 0;
+  for (int p = 0; p < nprocs_1; ++p) {
+    const int64_t cnt = local_n1_proc[p], st = local_1_start_proc[p];
+    if (cnt < 0 || st < 0 || st + cnt > N1) return ACCFFT_AMD_ERR_INVALID;
+    rk.a[p] = cnt;
+    rk.b[p] = st;
+    rk.pre[p + 1] = rk.pre[p] + cnt;
+    if (cnt * n_tuples > maxlen) maxlen = cnt * n_tuples;
+  }
+  const int64_t segs = (int64_t)nprocs_1 * howmany * local_n0;
+  if (segs == 0 || maxlen == 0) return ACCFFT_AMD_OK;
+  if (!send_recv || !data) return ACCFFT_AMD_ERR_NULL;
+  if (segs > 0x7fffffffLL) return ACCFFT_AMD_ERR_OVERFLOW;
+  hipLaunchKernelGGL(pencil_pack_kernel, dim3((unsigned)segs, chunks_for(maxlen * elem_size)),
+                     dim3(TPB), 0, (hipStream_t)stream, (int64_t)howmany, (int64_t)local_n0,
+                     n_tuples, N1, idist, rk, (const char*)data, (char*)send_recv, elem_size, scatter);
+  return hipGetLastError() == hipSuccess ? ACCFFT_AMD_OK : ACCFFT_AMD_ERR_HIP;
+}
+
+
 }  // namespace
+
+// The inverse of accfft_amd_pencil_pack (used by the pencil share's inverse
+// transpose): packed segment (proc, h, i) goes back to row i of batch h,
+// columns [start[proc], start[proc] + cnt[proc]).
+int afft_pencil_scatter(int nprocs, int howmany, int local_n0, int64_t n_tuples,
+                        const int64_t* cnt, const void* packed, void* data, int64_t idist,
+                        int64_t N1, const int64_t* start, int elem_size, void* stream) {
+  return pack_cols(1, nprocs, howmany, local_n0, n_tuples, cnt, const_cast<void*>(packed), data,
+                   idist, N1, start, elem_size, stream);
+}
 
 extern "C" {
 
@@ -236,29 +278,8 @@ int accfft_amd_pencil_pack(int nprocs_1, int howmany, int local_n0, int64_t n_tu
                            const int64_t* local_n1_proc, void* send_recv, const void* data,
                            int64_t idist, int64_t N1, const int64_t* local_1_start_proc,
                            int elem_size, void* stream) {
-  if (nprocs_1 < 1 || nprocs_1 > MAXP || howmany < 0 || local_n0 < 0 || n_tuples < 0 ||
-      elem_size <= 0 || idist < 0 || N1 < 0)
-    return ACCFFT_AMD_ERR_INVALID;
-  if (!local_n1_proc || !local_1_start_proc) return ACCFFT_AMD_ERR_NULL;
-  RankTab rk{};
-  int64_t maxlen = 0;
-  rk.pre[0] = 0;
-  for (int p = 0; p < nprocs_1; ++p) {
-    const int64_t cnt = local_n1_proc[p], st = local_1_start_proc[p];
-    if (cnt < 0 || st < 0 || st + cnt > N1) return ACCFFT_AMD_ERR_INVALID;
-    rk.a[p] = cnt;
-    rk.b[p] = st;
-    rk.pre[p + 1] = rk.pre[p] + cnt;
-    if (cnt * n_tuples > maxlen) maxlen = cnt * n_tuples;
-  }
-  const int64_t segs = (int64_t)nprocs_1 * howmany * local_n0;
-  if (segs == 0 || maxlen == 0) return ACCFFT_AMD_OK;
-  if (!send_recv || !data) return ACCFFT_AMD_ERR_NULL;
-  if (segs > 0x7fffffffLL) return ACCFFT_AMD_ERR_OVERFLOW;
-  hipLaunchKernelGGL(pencil_pack_kernel, dim3((unsigned)segs, chunks_for(maxlen * elem_size)),
-                     dim3(TPB), 0, (hipStream_t)stream, (int64_t)howmany, (int64_t)local_n0,
-                     n_tuples, N1, idist, rk, (const char*)data, (char*)send_recv, elem_size);
-  return hipGetLastError() == hipSuccess ? ACCFFT_AMD_OK : ACCFFT_AMD_ERR_HIP;
+  return pack_cols(0, nprocs_1, howmany, local_n0, n_tuples, local_n1_proc, send_recv,
+                   const_cast<void*>(data), idist, N1, local_1_start_proc, elem_size, stream);
 }
 
 int accfft_amd_pencil_unpack(int nprocs_0, int howmany, const int64_t* local_0_start_proc,

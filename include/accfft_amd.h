@@ -126,6 +126,46 @@ typedef int (*accfft_amd_exchange_fn)(const void* send, void* recv, int64_t bloc
                                       int nprocs, int direction, void* stream, void* user);
 int accfft_amd_plan_set_exchange(accfft_amd_plan* plan, accfft_amd_exchange_fn fn,
                                  void* user);
+/* One rank's share of accfft_plan_dft_3d_r2c_gpu(f) on a p0 x p1 grid at
+ * coordinates (c0, c1) (src/accfft_gpu.cpp:77-442, the pencil branch
+ * :354-375; execute :471-539 forward, :540-623 inverse): n is the global
+ * size; input [isize0][isize1][n2] real, output [n0][osize1][osize2]
+ * complex, sizes and starts as accfft_local_size_dft_r2c_gpu
+ * (src/accfft_common.cpp:98-192, ceil-block splits :49-91, any n and any
+ * grid: ragged last blocks and empty ranks included).  execute_r2c runs z
+ * r2c; T_plan_1 on the row communicator (the p1 ranks of this c0:
+ * memcpy_v1_h1 pack of the z column ranges, exchange, memcpy_v1_h2 unpack,
+ * src/transpose_gpu.cpp:1803-2130) to [isize0][n1][osize2]; y c2c; T_plan_2
+ * on the column communicator (the p0 ranks of this c1: the y row ranges
+ * packed, exchanged, received straight into the output rows of each
+ * sender's x range); x c2c.  execute_c2r runs the inverse (x c2c in place
+ * on its input, which it overwrites).  The exchanges go through the
+ * plan's exchange_v hook; with none each exchange is the identity (the
+ * packed blocks taken as received), allowed only where every block sent
+ * has the size of the block received from that peer (else execute
+ * returns ACCFFT_AMD_ERR_UNSUPPORTED).  p0, p1 <= 64; out of place; lines
+ * must fit one LDS tile (FP32 <= 10240, FP64 <= 5120 points, powers of two
+ * up to 4096 in FP64); full xyz only. */
+int accfft_amd_plan_dft_3d_r2c_pencil(const int n[3], int p0, int p1, int c0, int c1,
+                                      void* data, void* data_out, int fp, unsigned flags,
+                                      accfft_amd_plan** plan);
+/* Exchange hook of a pencil plan: the MPI_Alltoallv of the reference's
+ * pencil transposes (src/transpose_gpu.cpp:2058, :1701) on communicator
+ * comm (0 = row: the p1 ranks sharing c0, ordered by c1, T_plan_1 / _1i;
+ * 1 = column: the p0 ranks sharing c1, ordered by c0, T_plan_2 / _2i).
+ * send_bytes[q] bytes at send + send_offsets[q] go to peer q;
+ * recv_bytes[q] bytes from peer q land at recv + recv_offsets[q] (host
+ * arrays of nprocs entries, device buffers), on the given stream (a
+ * hipStream_t); direction -1 in execute_r2c, +1 in execute_c2r.  Called on
+ * every rank of the communicator, also where this rank's share is empty.
+ * Returns a status.  A RCCL send/recv group over device buffers is the
+ * intended implementation. */
+typedef int (*accfft_amd_exchange_v_fn)(int comm, const void* send, const int64_t* send_bytes,
+                                        const int64_t* send_offsets, void* recv,
+                                        const int64_t* recv_bytes, const int64_t* recv_offsets,
+                                        int nprocs, int direction, void* stream, void* user);
+int accfft_amd_plan_set_exchange_v(accfft_amd_plan* plan, accfft_amd_exchange_v_fn fn,
+                                   void* user);
 /* accfft_destroy_plan_gpu(f) (src/accfft_gpu.cpp:1150-1192).  Also frees the
  * plan object itself (the reference leaks it). */
 int accfft_amd_destroy_plan(accfft_amd_plan* plan);
@@ -133,7 +173,7 @@ int accfft_amd_destroy_plan(accfft_amd_plan* plan);
 int accfft_amd_plan_set_stream(accfft_amd_plan* plan, void* stream);
 /* Plan facts the drop-in wrappers mirror into accfft_plan_gpu(f):
  * n, isize, istart, osize, ostart (3 each), alloc_max, fp, kind (1 = r2c,
- * 2 = c2c, 3 = r2c slab share), inplace. */
+ * 2 = c2c, 3 = r2c slab share, 4 = r2c pencil share), inplace. */
 int accfft_amd_plan_info(const accfft_amd_plan* plan, int n[3], int isize[3],
                          int istart[3], int osize[3], int ostart[3],
                          int64_t* alloc_max, int* fp, int* kind, int* inplace);
