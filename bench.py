@@ -369,6 +369,8 @@ def run_wide(steps):
         f"round trip {t_rt:.3f} ms, rel err {err:.2e}")
     del u, h
     torch.cuda.empty_cache()
+    out.update(run_pencil_share(steps, g))
+    torch.cuda.empty_cache()
     # ---- 1024^3 round trip, plain and split-BF16
     n = (1024, 1024, 1024)
     x = torch.randn(n, generator=g, device=dev)
@@ -389,6 +391,47 @@ def run_wide(steps):
                                        "pass_model_frac": round(alg / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
         log(f"  1024^3 {name}: round trip {t:.3f} ms, rel err {e:.2e}")
     return out
+
+
+def run_pencil_share(steps, g):
+    """One GPU's share of a 2048^3 FP32 r2c on a 4 x 2 pencil grid, rank
+    (0, 0): [512][1024][2048] real -> [2048][512][513] complex (the half
+    spectrum's 1025 z columns split 513 / 512, so the row transposes are
+    ragged).  The exchange hook returns at once (the other ranks live on
+    other GPUs; the received blocks hold stale data), so the time is the
+    local part the reference reports in timer[4]: z r2c, the row pack and
+    unpack, y c2c, the column pack, x c2c (and the inverse)."""
+    import torch
+    import accfft_amd as af
+
+    dev = torch.device("cuda:0")
+    N, grid = (2048, 2048, 2048), (4, 2)
+    p = af.Plan("r2c_pencil", N, None, None, af.F32, af.ACCFFT_ESTIMATE, grid=grid,
+                coords=(0, 0))
+    n0, n1, _ = p.isize
+    _, ny, nz = p.osize
+    Nh = N[2] // 2 + 1
+    u = torch.randn((n0, n1, N[2]), generator=g, device=dev)
+    h = torch.empty(p.osize, dtype=torch.complex64, device=dev)
+    back = torch.empty_like(u)
+    p.set_exchange_v(lambda *a: 0)
+    t_fwd = _timeit(lambda: p.execute_r2c(u, h), steps)
+    t_inv = _timeit(lambda: p.execute_c2r(h, back), steps)
+    p.destroy()
+    R = n0 * n1 * N[2] * 4
+    C1 = n0 * n1 * Nh * 8  # [n0][n1][Nh]
+    C2 = n0 * N[1] * nz * 8  # [n0][N1][nz] = [N0][ny][nz]
+    model = R + 3 * C1 + 8 * C2  # z r2c, pack+unpack, y, pack, x: each element read and written
+    fused = R + C1 + 4 * C2  # the z, y and x passes alone (packs in their stores)
+    res = {"pencil_2048_4x2_rank00_f32": {
+        "fwd_ms": round(t_fwd, 4), "inv_ms": round(t_inv, 4),
+        "fwd_pass_model_bytes": model,
+        "fwd_frac_pass_model": round(model / (t_fwd * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "fwd_frac_fft_passes_only": round(fused / (t_fwd * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "exchange": "excluded (hook returns at once)"}}
+    log(f"  pencil 2048^3 4x2 rank (0,0) [{n0}][{n1}][{N[2]}] -> [{N[0]}][{ny}][{nz}]: "
+        f"fwd {t_fwd:.3f} ms, inv {t_inv:.3f} ms")
+    return res
 
 
 def run_ops(steps):

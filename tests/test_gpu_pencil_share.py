@@ -82,8 +82,10 @@ class World:
 
 
 def _run_ranks(work):
-    """Runs work(rank) for every rank in its own thread on its own stream."""
+    """Runs work(rank) for every rank in its own thread on its own stream,
+    after everything queued so far (inputs written on the default stream)."""
     import torch
+    torch.cuda.synchronize()
     errs = []
 
     def body(rank, fn):
@@ -226,3 +228,89 @@ def test_pencil_plan_arguments(dev):
     with pytest.raises(af.AccfftError, match="wrong kind"):
         p.set_exchange(None)
     p.destroy()
+
+
+def test_pencil_share_2048_fp32_full_grid(dev):
+    """2048^3 FP32 on a 4 x 2 grid, all eight ranks at their real local
+    shapes ([512][1024][2048] real -> [2048][512][513 | 512] complex: the
+    row transposes are ragged) on the one GPU (about 180 GB).  Forward of
+    cos(2 pi (kx x + ky y + kz z) / 2048): the one non-zero bin of the half
+    spectrum, N^3 / 2 at (kx, ky, kz), lies in the block of rank (2, 1) and
+    every other bin of every rank is zero; round trip of N(0,1) blocks."""
+    import torch
+    import accfft_amd as af
+    N, grid = (2048, 2048, 2048), (4, 2)
+    kx, ky, kz = 3, 1500, 700
+    print(f"pencil share {N} grid {grid} FP32, all ranks", flush=True)
+    hip = _hip()
+    cs = 8
+    Nh = N[2] // 2 + 1
+    totals, plans, ins, outs = {}, {}, {}, {}
+    for c0, c1 in itertools.product(range(grid[0]), range(grid[1])):
+        p = af.Plan("r2c_pencil", N, None, None, af.F32, af.ACCFFT_ESTIMATE, grid=grid,
+                    coords=(c0, c1))
+        n0, n1, _ = p.isize
+        ny, nz = p.osize[1], p.osize[2]
+        totals[(c0, c1, 0, -1)] = (n0 * n1 * Nh * cs, n0 * N[1] * nz * cs)
+        totals[(c0, c1, 1, -1)] = (n0 * N[1] * nz * cs, N[0] * ny * nz * cs)
+        totals[(c0, c1, 1, 1)] = totals[(c0, c1, 1, -1)][::-1]
+        totals[(c0, c1, 0, 1)] = totals[(c0, c1, 0, -1)][::-1]
+        xs, ys1 = p.istart[0], p.istart[1]
+        t = 2 * np.pi / N[0]
+        x = torch.arange(xs, xs + n0, device=dev, dtype=torch.float64)
+        y = torch.arange(ys1, ys1 + n1, device=dev, dtype=torch.float64)
+        z = torch.arange(N[2], device=dev, dtype=torch.float64)
+        a = ((kx * x).view(-1, 1) + (ky * y).view(1, -1)) % N[0] * t
+        b = (kz * z) % N[2] * t
+        u = torch.cos(a).float().unsqueeze(-1) * torch.cos(b).float()
+        u.sub_(torch.sin(a).float().unsqueeze(-1) * torch.sin(b).float())
+        ins[(c0, c1)] = u
+        outs[(c0, c1)] = torch.empty(p.osize, dtype=torch.complex64, device=dev)
+        plans[(c0, c1)] = p
+    try:
+        _full_grid_checks(dev, N, grid, (kx, ky, kz), hip, totals, plans, ins, outs)
+    finally:  # release the ~180 GB also when a check fails
+        for p in plans.values():
+            p.destroy()
+        ins.clear()
+        outs.clear()
+        torch.cuda.empty_cache()
+
+
+def _full_grid_checks(dev, N, grid, k, hip, totals, plans, ins, outs):
+    import torch
+    kx, ky, kz = k
+    world = World(grid, hip, totals)
+    for r, p in plans.items():
+        p.set_exchange_v(world.hook(*r))
+    _run_ranks({r: (lambda r=r: plans[r].execute_r2c(ins[r], outs[r])) for r in plans})
+    peak = float(N[0]) * N[1] * N[2] / 2
+    hit = []
+    rest2 = 0.0
+    for (c0, c1), p in plans.items():
+        o = outs[(c0, c1)]
+        ys, zs = p.ostart[1], p.ostart[2]
+        ny, nz = p.osize[1], p.osize[2]
+        if ys <= ky < ys + ny and zs <= kz < zs + nz:
+            v = o[kx, ky - ys, kz - zs].item()
+            assert abs(v - peak) / peak < 1e-5, (c0, c1, v)
+            o[kx, ky - ys, kz - zs] = 0
+            hit.append((c0, c1))
+        rest2 += (o.abs().double().norm() ** 2).item()
+    assert hit == [(2, 1)], hit
+    rest = rest2 ** 0.5 / peak
+    print(f"  closed form: rest of the spectrum {rest:.2e} of the peak", flush=True)
+    assert rest < 1e-5
+    # round trip of N(0,1) blocks, each rank's result written over its input
+    for i, r in enumerate(plans):
+        g = torch.Generator(device=dev).manual_seed(90 + i)
+        ins[r].copy_(torch.randn(ins[r].shape, generator=g, device=dev))
+    _run_ranks({r: (lambda r=r: plans[r].execute_r2c(ins[r], outs[r])) for r in plans})
+    _run_ranks({r: (lambda r=r: plans[r].execute_c2r(outs[r], ins[r])) for r in plans})
+    scale = float(np.prod(N))
+    for i, r in enumerate(plans):
+        g = torch.Generator(device=dev).manual_seed(90 + i)
+        want = torch.randn(ins[r].shape, generator=g, device=dev)
+        err = ((ins[r] / scale - want).double().norm() / want.double().norm()).item()
+        assert err < 1e-5, (r, err)
+        del want
