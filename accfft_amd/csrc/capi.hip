@@ -359,6 +359,8 @@ int accfft_amd_plan_dft_3d_r2c_pencil(const int n[3], int p0, int p1, int c0, in
   if (bad_grid(n, p0, p1, c0, c1)) return ACCFFT_AMD_ERR_INVALID;
   if (p0 > 64 || p1 > 64) return ACCFFT_AMD_ERR_UNSUPPORTED;
   if (data && data == data_out) return ACCFFT_AMD_ERR_UNSUPPORTED;  // out of place only
+  if (flags & (ACCFFT_AMD_SPLIT_BF16 | ACCFFT_AMD_SPLIT_BF16X3 | ACCFFT_AMD_SPLIT_FP16))
+    return ACCFFT_AMD_ERR_UNSUPPORTED;  // FP32 passes only
   accfft_amd_plan* p = nullptr;
   int st = make_plan(4, n, data, data_out, fp, flags, &p);
   if (st) return st;

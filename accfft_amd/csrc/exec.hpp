@@ -631,9 +631,7 @@ struct Ops {
       auto az = zargs(in, zbuf, n0 * N1, N2, Np);
       if ((st = run_pass<T>(p, "z", false, M_R2C, -1, az, tab(p, 2), 0))) return st;
       auto ay = sargs(zbuf, send, n0 * Nh, Nh, N1 * Np, n1 * Nh, Np, Nh);
-      ay.oshift = ilog2(n1);
-      ay.Bout = n0 * n1 * Nh;
-      if (P == 1) ay.oshift = 0;
+      if (P > 1) out_blocks(ay, n1, n0 * n1 * Nh);
       if ((st = run_pass<T>(p, "y", true, M_C2C, -1, ay, tab(p, 1), 1))) return st;
     }
     if (p->exchange && (st = exchange(p, send, out, (int64_t)(n0 * n1 * Nh * sizeof(C)), -1)))
@@ -673,9 +671,7 @@ struct Ops {
       return run_zy(p, +1, false, zy_args(p, ybuf, out, n0, Np, N2));
     }
     auto ay = sargs(recv, ybuf, n0 * Nh, Nh, n1 * Nh, N1 * Np, Nh, Np);
-    ay.ishift = ilog2(n1);
-    ay.Bin = n0 * n1 * Nh;
-    if (P == 1) ay.ishift = 0;
+    if (P > 1) in_blocks(ay, n1, n0 * n1 * Nh);
     if ((st = run_pass<T>(p, "y", true, M_C2C, +1, ay, tab(p, 1), 4))) return st;
     auto az = zargs(ybuf, out, n0 * N1, Np, N2);
     return run_pass<T>(p, "z", false, M_C2R, +1, az, tab(p, 2), 5);
@@ -747,11 +743,89 @@ struct Ops {
     const int64_t nz = p->osize[2];
     const size_t b = (size_t)std::max(n0 * n1 * Nh, n0 * N1 * nz) * sizeof(C);
     const bool hook = p->exchange_v != nullptr;
+    // separate copies: 0 the z output and y buffer, 1 the send blocks, 2 the
+    // receive blocks of the row exchange; fused: 0 the row exchange's blocks
+    // of the z side, 1 received blocks, 2 the y pass's exchange blocks
+    bool need[3];
+    if (pencil_fused(p)) {
+      need[0] = true;
+      need[1] = hook && (p->p0 > 1 || p->p1 > 1);
+      need[2] = p->p1 > 1 || (p->p0 > 1 && hook);
+    } else {
+      need[0] = true;
+      need[1] = p->p1 > 1 || (p->p0 > 1 && hook);
+      need[2] = p->p1 > 1 && hook;
+    }
     int st;
-    if ((st = ensure_ws(p, 0, b))) return st;
-    if ((p->p1 > 1 || (p->p0 > 1 && hook)) && (st = ensure_ws(p, 1, b))) return st;
-    if (p->p1 > 1 && hook && (st = ensure_ws(p, 2, b))) return st;
+    for (int i = 0; i < 3; ++i)
+      if (need[i] && (st = ensure_ws(p, i, b))) return st;
     return ACCFFT_AMD_OK;
+  }
+  // Whether the copies ride on the FFT passes: the y rows split in equal
+  // power-of-two blocks over both communicators (or one rank), so that the
+  // y pass loads the row exchange's received blocks and stores the column
+  // exchange's send blocks as row blocks (FftArgs::ishift / oshift), and
+  // the z pass stores (loads, inverse) the row exchange's column blocks
+  // (FftArgs::zcol).  The forward then moves R + 5 C bytes like a 1 x 1
+  // plan; other splits run the separate pack / unpack copies.
+  static bool pencil_fused(const accfft_amd_plan* p) {
+    auto even = [&](int P) {
+      if (P == 1) return true;
+      const int b = p->n[1] / P;
+      return p->n[1] % P == 0 && (b & (b - 1)) == 0;
+    };
+    return even(p->p0) && even(p->p1);
+  }
+  static int r2c_pencil_fused(accfft_amd_plan* p, const void* in, void* out, const Xchg& t1,
+                              const Xchg& t2) {
+    const int64_t N1 = p->n[1], N2 = p->n[2], Nh = N2 / 2 + 1;
+    const int64_t n0 = p->isize[0], n1 = p->isize[1], ny = p->osize[1], nz = p->osize[2];
+    const int P0 = p->p0, P1 = p->p1;
+    const bool hook = p->exchange_v != nullptr;
+    int st;
+    // z r2c into the row exchange's send blocks [q][n0 n1][zn_q]
+    auto az = zargs(in, p->ws[0], n0 * n1, N2, Nh);
+    if (P1 > 1) az.zcol = p->zn[0];
+    if ((st = run_pass<T>(p, "z", false, M_R2C, -1, az, tab(p, 2), 0))) return st;
+    const void* yin = p->ws[0];
+    if (P1 > 1 && (st = pencil_exchange(p, 0, t1, p->ws[0], p->ws[1], -1, &yin))) return st;
+    // y c2c from the received row blocks [q][n0][y1n_q][nz] into the column
+    // exchange's send blocks [q][n0][yn_q][nz] (the output for one rank)
+    void* yout = (P0 > 1 && hook) ? p->ws[2] : out;
+    auto ay = sargs(yin, yout, n0 * nz, nz, (P1 > 1 ? n1 : N1) * nz, (P0 > 1 ? ny : N1) * nz,
+                    nz, nz);
+    if (P1 > 1) in_blocks(ay, n1, n0 * n1 * nz);
+    if (P0 > 1) out_blocks(ay, ny, n0 * ny * nz);
+    if ((st = run_pass<T>(p, "y", true, M_C2C, -1, ay, tab(p, 1), 1))) return st;
+    const void* got;
+    if (P0 > 1 && (st = pencil_exchange(p, 1, t2, yout, out, -1, &got))) return st;
+    auto ax = sargs(out, out, ny * nz, ny * nz, 0, 0, ny * nz, ny * nz);
+    return run_pass<T>(p, "x", true, M_C2C, -1, ax, tab(p, 0), 2);
+  }
+  static int c2r_pencil_fused(accfft_amd_plan* p, void* in, void* out, const Xchg& t1,
+                              const Xchg& t2) {
+    const int64_t N1 = p->n[1], N2 = p->n[2], Nh = N2 / 2 + 1;
+    const int64_t n0 = p->isize[0], n1 = p->isize[1], ny = p->osize[1], nz = p->osize[2];
+    const int P0 = p->p0, P1 = p->p1;
+    int st;
+    auto ax = sargs(in, in, ny * nz, ny * nz, 0, 0, ny * nz, ny * nz);
+    if ((st = run_pass<T>(p, "x", true, M_C2C, +1, ax, tab(p, 0), 3))) return st;
+    const void* yin = in;
+    if (P0 > 1 && (st = pencil_exchange(p, 1, t2, in, p->ws[1], +1, &yin))) return st;
+    // y c2c from the received column blocks into the row exchange's send
+    // blocks (the z pass's input for one rank)
+    void* yout = P1 > 1 ? p->ws[2] : p->ws[0];
+    auto ay = sargs(yin, yout, n0 * nz, nz, (P0 > 1 ? ny : N1) * nz, (P1 > 1 ? n1 : N1) * nz,
+                    nz, nz);
+    if (P0 > 1) in_blocks(ay, ny, n0 * ny * nz);
+    if (P1 > 1) out_blocks(ay, n1, n0 * n1 * nz);
+    if ((st = run_pass<T>(p, "y", true, M_C2C, +1, ay, tab(p, 1), 4))) return st;
+    const void* zin = yout;
+    if (P1 > 1 && (st = pencil_exchange(p, 0, t1, yout, p->ws[0], +1, &zin))) return st;
+    // z c2r from the received column blocks [q][n0 n1][zn_q]
+    auto az = zargs(zin, out, n0 * n1, Nh, N2);
+    if (P1 > 1) az.zcol = p->zn[0];
+    return run_pass<T>(p, "z", false, M_C2R, +1, az, tab(p, 2), 5);
   }
   static int r2c_pencil(accfft_amd_plan* p, const void* in, void* out) {
     const int64_t N1 = p->n[1], N2 = p->n[2], Nh = N2 / 2 + 1;
@@ -764,6 +838,7 @@ struct Ops {
       return ACCFFT_AMD_ERR_UNSUPPORTED;
     int st;
     if ((st = pencil_ws(p))) return st;
+    if (pencil_fused(p)) return r2c_pencil_fused(p, in, out, t1, t2);
     void* A = p->ws[0];
     auto az = zargs(in, A, n0 * n1, N2, Nh);
     if ((st = run_pass<T>(p, "z", false, M_R2C, -1, az, tab(p, 2), 0))) return st;
@@ -807,6 +882,7 @@ struct Ops {
       return ACCFFT_AMD_ERR_UNSUPPORTED;
     int st;
     if ((st = pencil_ws(p))) return st;
+    if (pencil_fused(p)) return c2r_pencil_fused(p, in, out, t1, t2);
     void* A = p->ws[0];
     auto ax = sargs(in, in, ny * nz, ny * nz, 0, 0, ny * nz, ny * nz);
     if ((st = run_pass<T>(p, "x", true, M_C2C, +1, ax, tab(p, 0), 3))) return st;
@@ -838,6 +914,19 @@ struct Ops {
     int s = 0;
     while ((int64_t(1) << s) < v) ++s;
     return s;
+  }
+  // Rows of a strided pass's input / output in blocks of `rows` (a power of
+  // two) at distance B (FftArgs::ishift / oshift); blocks of one row are
+  // shift 0 with the row stride B.
+  static void in_blocks(FftArgs<T>& a, int64_t rows, int64_t B) {
+    if (rows == 1) a.Sin = B;
+    else a.ishift = ilog2(rows);
+    a.Bin = B;
+  }
+  static void out_blocks(FftArgs<T>& a, int64_t rows, int64_t B) {
+    if (rows == 1) a.Sout = B;
+    else a.oshift = ilog2(rows);
+    a.Bout = B;
   }
 
   // ---------------------------------------------------------- adjoints
@@ -1002,8 +1091,7 @@ struct Ops {
       const int P = p->nprocs;
       const int64_t n0 = N0 / P, n1 = N1 / P, Np = ws_pitch(Nh);
       auto ay = sargs(nullptr, nullptr, n0 * Nh, Nh, N1 * Np, n1 * Nh, Np, Nh);
-      ay.oshift = P == 1 ? 0 : ilog2(n1);
-      ay.Bout = n0 * n1 * Nh;
+      if (P > 1) out_blocks(ay, n1, n0 * n1 * Nh);
       if (!zy_fused(p)) passes.push_back({1, 1, ay, n0 * N1 * Np, N0 * n1 * Nh});
       passes.push_back({0, 2, sargs(nullptr, nullptr, n1 * Nh, n1 * Nh, 0, 0, n1 * Nh, n1 * Nh),
                         N0 * n1 * Nh, 0});

@@ -53,7 +53,24 @@ struct FftArgs {
   // This is the pack / unpack of a slab exchange fused into the y pass.
   int ishift, oshift;
   int64_t Bin, Bout;
+  // z R2C store / C2R load only: the half-spectrum columns in blocks of
+  // zcol (the last one cut at n/2+1), block q of every line stored together
+  // at q * zcol * nlines as [nlines][width of block q]; 0 = plain rows.
+  // This is the pack / unpack of a pencil share's row exchange fused into
+  // the z pass.
+  int64_t zcol;
 };
+
+// Element k of half-spectrum line g of a z pass (see FftArgs::zcol).
+template <typename T>
+__device__ __forceinline__ int64_t zcol_off(const FftArgs<T>& a, int64_t g, int64_t dist, int k,
+                                            int nh) {
+  if (a.zcol == 0) return g * dist + k;
+  const int zc = (int)a.zcol;  // < n / 2 + 1: 32-bit division
+  const int s = k / zc * zc;
+  const int w = (nh - s) < zc ? (nh - s) : zc;
+  return (int64_t)s * a.nlines + g * w + (k - s);
+}
 
 __device__ __forceinline__ int64_t row_off(int r, int64_t S, int shift, int64_t B) {
   if (shift == 0) return (int64_t)r * S;
@@ -247,8 +264,8 @@ __global__ void __launch_bounds__(ZT > 256 ? ZT : 256) zfft_kernel(FftArgs<T> a)
         if (q < NHT) {
           const int p = q / NH, k = q % NH;
           const int64_t ga = 2 * (l0 + p), gb = ga + 1;
-          if (ga < nl) A[j] = ldc<T>(a.in, ga * a.idist + k);
-          if (gb < nl) B[j] = ldc<T>(a.in, gb * a.idist + k);
+          if (ga < nl) A[j] = ldc<T>(a.in, zcol_off(a, ga, a.idist, k, NH));
+          if (gb < nl) B[j] = ldc<T>(a.in, zcol_off(a, gb, a.idist, k, NH));
         }
       }
       stage_twiddles<T, TWN, TPB>(const_cast<cpx<T>*>(twl), a.tw, t);
@@ -311,8 +328,8 @@ __global__ void __launch_bounds__(ZT > 256 ? ZT : 256) zfft_kernel(FftArgs<T> a)
       const int64_t ga = 2 * (l0 + p), gb = ga + 1;
       const int k = i < nh ? i : n - i;
       cpx<T> A = {(T)0, (T)0}, B = {(T)0, (T)0};
-      if (ga < nl) A = ldc<T>(a.in, ga * a.idist + k);
-      if (gb < nl) B = ldc<T>(a.in, gb * a.idist + k);
+      if (ga < nl) A = ldc<T>(a.in, zcol_off(a, ga, a.idist, k, nh));
+      if (gb < nl) B = ldc<T>(a.in, zcol_off(a, gb, a.idist, k, nh));
       if (k == 0 || 2 * k == n) {
         A.y = (T)0;
         B.y = (T)0;
@@ -381,7 +398,7 @@ __global__ void __launch_bounds__(ZT > 256 ? ZT : 256) zfft_kernel(FftArgs<T> a)
       if ((rl & 1) == 0) o = {h * (z1.x + z2.x), h * (z1.y - z2.y)};
       else o = {h * (z1.y + z2.y), h * (z2.x - z1.x)};
       if (a.adjw && k != 0 && 2 * k != n) o = cscale(o, (T)2);
-      stc<T>(a.out, g * a.odist + k, o);
+      stc<T>(a.out, zcol_off(a, g, a.odist, k, n / 2 + 1), o);
     }
   } else {  // real outputs
     if constexpr (MODE == M_DERIV && STATIC) {

@@ -7,7 +7,10 @@ does in the reference's pencil transposes, src/transpose_gpu.cpp:2058).
 Each rank's forward output is checked against the oracle's global r2c
 restricted to its block [N0][ys:ys+ny][zs:zs+nz], its inverse against the
 oracle's global c2r restricted to [xs:xs+n0][y1s:y1s+n1]: ragged splits,
-ranks with empty shares, communicators of one rank (no exchange issued)."""
+ranks with empty shares, communicators of one rank (no exchange issued),
+and both forms of the copies (separate pack / unpack kernels, or carried
+by the z and y passes when the y rows split evenly in power-of-two
+blocks)."""
 import ctypes
 import itertools
 import threading
@@ -109,7 +112,12 @@ def _run_ranks(work):
 CASES = [((8, 12, 10), (2, 3), "f64"), ((9, 10, 7), (4, 2), "f64"), ((6, 8, 16), (1, 4), "f64"),
          ((10, 6, 8), (3, 1), "f64"), ((8, 12, 6), (2, 3), "f64"), ((7, 5, 12), (3, 5), "f64"),
          ((16, 16, 18), (4, 4), "f64"), ((32, 24, 40), (2, 2), "f64"), ((5, 4, 3), (1, 1), "f64"),
-         ((64, 48, 80), (2, 3), "f32"), ((128, 96, 160), (4, 2), "f32")]
+         ((64, 48, 80), (2, 3), "f32"), ((128, 96, 160), (4, 2), "f32"),
+         # y rows split in equal power-of-two blocks: the copies ride on the
+         # z and y passes (ragged z column blocks, an empty z block, one-row
+         # y blocks with empty x ranks)
+         ((8, 16, 20), (2, 4), "f64"), ((16, 64, 36), (4, 4), "f64"), ((9, 12, 6), (3, 3), "f64"),
+         ((7, 8, 10), (8, 8), "f64"), ((64, 64, 62), (2, 4), "f32")]
 
 
 @pytest.mark.parametrize("N,grid,prec", CASES)

@@ -15,7 +15,7 @@ from conftest import rel_l2, rng
 pytestmark = pytest.mark.gpu
 
 CASES = [((8, 16, 10), 4), ((16, 32, 24), 4), ((64, 64, 64), 8), ((24, 16, 33), 2),
-         ((12, 10, 6), 1)]
+         ((12, 10, 6), 1), ((8, 4, 6), 4)]  # the last: one y row per rank
 
 
 def _plan(N, P, fp, a, h):
