@@ -599,6 +599,10 @@ int accfft_amd_inv_laplace_noalloc(accfft_amd_plan* p, void* out, const void* A,
   if (p->kind != 1) return ACCFFT_AMD_ERR_NOT_BAKED;
   const int64_t cb = (int64_t)p->n[0] * p->n[1] * (p->n[2] / 2 + 1) * 2 * elem_of(p->fp);
   if (cb > (int64_t)ACCFFT_AMD_NOALLOC_MAX_BYTES) return ACCFFT_AMD_ERR_SCRATCH;
+  // lines longer than one LDS tile run through workspaces of the plan's own
+  for (int ax = 0; ax < 3; ++ax)
+    if (p->fp == ACCFFT_AMD_F32 ? p->tf[ax].longl : p->td[ax].longl)
+      return ACCFFT_AMD_ERR_UNSUPPORTED;
   int device = -1;
   if (hipGetDevice(&device) != hipSuccess) return ACCFFT_AMD_ERR_HIP;
   if (device != p->device) return ACCFFT_AMD_ERR_INVALID;

@@ -176,6 +176,49 @@ inline int prof_launch(accfft_amd_plan* p, const char* name, double bytes, F&& l
 // lines per complex FFT), as global radix passes (fft_long.hpp) through
 // workspaces 2 and 3.  The fused modes, strided real passes, exchange-block
 // maps and the adjoint weights have no long-line form.
+// nlines long lines from (src, gs) to (dst, gd) (src may equal dst): the
+// radix passes of long_c2c through ws, or Bluestein's algorithm in chunks
+// of lines whose two convolution buffers (workspaces 4, 5) hold at most
+// 256 MiB each.
+template <typename T>
+inline int long_lines(accfft_amd_plan* p, const AxisTab<T>& tab, const cpx<T>* src, GGeom gs,
+                      cpx<T>* dst, GGeom gd, cpx<T>* ws, int64_t nlines, int sign,
+                      int lines_fast) {
+  using C = cpx<T>;
+  if (!tab.bm)
+    return status_of(long_c2c<T>(src, gs, dst, gd, ws, nlines, tab.sc, tab.tw_full, sign,
+                                 lines_fast, p->stream));
+  const int N = tab.n, M = tab.bm;
+  int64_t chunk = ((int64_t)256 << 20) / ((int64_t)M * (int64_t)sizeof(C));
+  if (chunk < 1) chunk = 1;
+  if (chunk > nlines) chunk = nlines;
+  int st;
+  if ((st = ensure_ws(p, 4, (size_t)chunk * M * sizeof(C))) ||
+      (st = ensure_ws(p, 5, (size_t)chunk * M * sizeof(C))))
+    return st;
+  C* a = (C*)p->ws[4];
+  C* w = (C*)p->ws[5];
+  const GGeom ga = {1, M, 1};
+  const dim3 b(GTPB);
+  for (int64_t l0 = 0; l0 < nlines; l0 += chunk) {
+    const int64_t nl = nlines - l0 < chunk ? nlines - l0 : chunk;
+    hipLaunchKernelGGL((blue_pre_kernel<T>), dim3(gblocks(nl * M)), b, 0, p->stream, src, gs, a,
+                       l0, nl, N, M, tab.bchirp, sign);
+    if ((st = status_of(hipGetLastError()))) return st;
+    if ((st = status_of(long_c2c<T>(a, ga, a, ga, w, nl, tab.bsc, tab.btw, -1, 0, p->stream))))
+      return st;
+    hipLaunchKernelGGL((blue_mul_kernel<T>), dim3(gblocks(nl * M)), b, 0, p->stream, a, nl, M,
+                       tab.bker[sign > 0 ? 1 : 0]);
+    if ((st = status_of(hipGetLastError()))) return st;
+    if ((st = status_of(long_c2c<T>(a, ga, a, ga, w, nl, tab.bsc, tab.btw, +1, 0, p->stream))))
+      return st;
+    hipLaunchKernelGGL((blue_post_kernel<T>), dim3(gblocks(nl * N)), b, 0, p->stream, a, dst, gd,
+                       l0, nl, N, M, tab.bchirp, sign);
+    if ((st = status_of(hipGetLastError()))) return st;
+  }
+  return ACCFFT_AMD_OK;
+}
+
 template <typename T>
 inline int long_pass(accfft_amd_plan* p, bool strided, int mode, int sign, const FftArgs<T>& a,
                      const AxisTab<T>& tab) {
@@ -195,15 +238,15 @@ inline int long_pass(accfft_amd_plan* p, bool strided, int mode, int sign, const
     }
     const int64_t L = a.nlines - 1;
     const int64_t extent = (L / gd.C) * gd.D + L % gd.C + (int64_t)(N - 1) * gd.S + 1;
-    if ((st = ensure_ws(p, 2, (size_t)extent * sizeof(C)))) return st;
-    return status_of(long_c2c<T>((const C*)a.in, gs, (C*)a.out, gd, (C*)p->ws[2], a.nlines,
-                                 tab.sc, tab.tw_full, sign, strided ? 1 : 0, p->stream));
+    if (!tab.bm && (st = ensure_ws(p, 2, (size_t)extent * sizeof(C)))) return st;
+    return long_lines<T>(p, tab, (const C*)a.in, gs, (C*)a.out, gd, (C*)p->ws[2], a.nlines, sign,
+                         strided ? 1 : 0);
   }
   if (strided || (mode != M_R2C && mode != M_C2R)) return ACCFFT_AMD_ERR_UNSUPPORTED;
   // z passes: line pairs packed into complex lines of length N (pitch N)
   const int64_t np = (a.nlines + 1) / 2;
   const size_t wb = (size_t)np * N * sizeof(C);
-  if ((st = ensure_ws(p, 2, wb)) || (st = ensure_ws(p, 3, wb))) return st;
+  if ((!tab.bm && (st = ensure_ws(p, 2, wb))) || (st = ensure_ws(p, 3, wb))) return st;
   C* z = (C*)p->ws[3];
   const GGeom gz = {1, N, 1};
   const dim3 b(GTPB);
@@ -211,9 +254,7 @@ inline int long_pass(accfft_amd_plan* p, bool strided, int mode, int sign, const
     hipLaunchKernelGGL((gpack_r2c_kernel<T>), dim3(gblocks(np * N)), b, 0, p->stream,
                        (const T*)a.in, a.idist, z, a.nlines, N);
     if ((st = status_of(hipGetLastError()))) return st;
-    if ((st = status_of(long_c2c<T>(z, gz, z, gz, (C*)p->ws[2], np, tab.sc, tab.tw_full, -1, 0,
-                                    p->stream))))
-      return st;
+    if ((st = long_lines<T>(p, tab, z, gz, z, gz, (C*)p->ws[2], np, -1, 0))) return st;
     hipLaunchKernelGGL((gsplit_r2c_kernel<T>), dim3(gblocks(np * (N / 2 + 1))), b, 0, p->stream,
                        z, (C*)a.out, a.odist, a.nlines, N);
     return status_of(hipGetLastError());
@@ -221,9 +262,7 @@ inline int long_pass(accfft_amd_plan* p, bool strided, int mode, int sign, const
   hipLaunchKernelGGL((gpack_c2r_kernel<T>), dim3(gblocks(np * (N / 2 + 1))), b, 0, p->stream,
                      (const C*)a.in, a.idist, z, a.nlines, N);
   if ((st = status_of(hipGetLastError()))) return st;
-  if ((st = status_of(long_c2c<T>(z, gz, z, gz, (C*)p->ws[2], np, tab.sc, tab.tw_full, +1, 0,
-                                  p->stream))))
-    return st;
+  if ((st = long_lines<T>(p, tab, z, gz, z, gz, (C*)p->ws[2], np, +1, 0))) return st;
   hipLaunchKernelGGL((gunpack_c2r_kernel<T>), dim3(gblocks(np * N)), b, 0, p->stream, z,
                      (T*)a.out, a.odist, a.nlines, N);
   return status_of(hipGetLastError());

@@ -41,6 +41,15 @@ struct AxisTab {
   int xg = -1;
   // lines longer than one LDS tile: global radix passes (fft_long.hpp)
   bool longl = false;
+  // long lines with a prime factor above 61: Bluestein's algorithm over
+  // power-of-two convolutions of length bm (0: not used); bchirp =
+  // exp(-i pi n^2 / n), bker[0 / 1] = the convolution kernel's spectrum / bm
+  // for the forward / inverse transform, btw and bsc the bm-point tables
+  int bm = 0;
+  DynSched bsc{};
+  const cpx<T>* bchirp = nullptr;
+  const cpx<T>* bker[2] = {nullptr, nullptr};
+  const cpx<T>* btw = nullptr;
 };
 
 // Lengths with a matrix-core (split-precision) schedule.

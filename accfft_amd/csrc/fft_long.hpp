@@ -203,6 +203,67 @@ __global__ void __launch_bounds__(GTPB)
   }
 }
 
+// Bluestein's algorithm for lines whose length N has a prime factor above
+// 61 (Bluestein 1970): with nk = (n^2 + k^2 - (k - n)^2) / 2,
+//   X[k] = c[k] sum_n (x[n] c[n]) h[k - n],  c[n] = exp(s i pi n^2 / N),
+//   h[m] = conj c[m],
+// a circular convolution of power-of-two length M >= 2N - 1: a = x c
+// zero-padded to M, a = IFFT_M(FFT_M(a) H), X = c a on k < N, where H is
+// the plan's FFT_M of h (circularly extended) divided by M.  chirp holds
+// exp(-i pi n^2 / N); the inverse transform (s = +1) uses its conjugate.
+template <typename T>
+__global__ void __launch_bounds__(GTPB)
+    blue_pre_kernel(const cpx<T>* __restrict__ src, GGeom gs, cpx<T>* __restrict__ a,
+                    int64_t line0, int64_t nl, int N, int M, const cpx<T>* __restrict__ chirp,
+                    int sign) {
+  const int64_t total = nl * M;
+  for (int64_t b = (int64_t)blockIdx.x * GTPB + threadIdx.x; b < total;
+       b += (int64_t)gridDim.x * GTPB) {
+    const int64_t l = b / M, n = b - l * M;
+    cpx<T> v = {(T)0, (T)0};
+    if (n < N) {
+      const cpx<T> x = src[gs.base(line0 + l) + n * gs.S];
+      v = sign < 0 ? cmul(x, chirp[n]) : cmulc(x, chirp[n]);
+    }
+    a[b] = v;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(GTPB)
+    blue_mul_kernel(cpx<T>* __restrict__ a, int64_t nl, int M, const cpx<T>* __restrict__ H) {
+  const int64_t total = nl * M;
+  for (int64_t b = (int64_t)blockIdx.x * GTPB + threadIdx.x; b < total;
+       b += (int64_t)gridDim.x * GTPB)
+    a[b] = cmul(a[b], H[b % M]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(GTPB)
+    blue_post_kernel(const cpx<T>* __restrict__ a, cpx<T>* __restrict__ dst, GGeom gd,
+                     int64_t line0, int64_t nl, int N, int M, const cpx<T>* __restrict__ chirp,
+                     int sign) {
+  const int64_t total = nl * N;
+  for (int64_t b = (int64_t)blockIdx.x * GTPB + threadIdx.x; b < total;
+       b += (int64_t)gridDim.x * GTPB) {
+    const int64_t l = b / N, k = b - l * N;
+    const cpx<T> v = a[l * M + k];
+    dst[gd.base(line0 + l) + k * gd.S] = sign < 0 ? cmul(v, chirp[k]) : cmulc(v, chirp[k]);
+  }
+}
+
+// Whether a length runs Bluestein's algorithm: a prime factor above 61.
+inline bool bluestein_length(int n) {
+  int big = 1, m = n;
+  for (int f = 2; (long)f * f <= m; ++f)
+    while (m % f == 0) {
+      big = f;
+      m /= f;
+    }
+  if (m > 1) big = m;  // the remaining factor is a prime above every one divided out
+  return big > 61;
+}
+
 inline unsigned gblocks(int64_t work) {
   int64_t b = (work + GTPB - 1) / GTPB;
   if (b > 65536) b = 65536;

@@ -53,11 +53,12 @@ struct accfft_amd_plan {
   // the y axis turns it off, -1 restores it)
   bool yfuse = false;
   std::vector<void*> dev_mem;  // table allocations
-  // 0, 1: spectra of the operators, adjoints and slab plans; 2, 3: the
-  // ping-pong buffers of lines longer than one LDS tile (fft_long.hpp)
-  static constexpr int NWS = 4;
-  void* ws[NWS] = {nullptr, nullptr, nullptr, nullptr};
-  size_t ws_bytes[NWS] = {0, 0, 0, 0};
+  // 0, 1: spectra of the operators, adjoints and slab plans (and 2 of the
+  // pencil plans); 2, 3: the ping-pong buffers of lines longer than one LDS
+  // tile (fft_long.hpp); 4, 5: Bluestein's convolution buffers
+  static constexpr int NWS = 6;
+  void* ws[NWS] = {};
+  size_t ws_bytes[NWS] = {};
   hipStream_t stream = nullptr;
   // stream of the previous call: when a call comes on another stream, that
   // stream first waits for everything already queued on the previous one

@@ -240,7 +240,9 @@ int accfft_amd_inv_laplace(accfft_amd_plan* plan, void* out, const void* A, doub
  * hipMalloc).  Same kernels and result as accfft_amd_inv_laplace, but the
  * intermediate spectrum lives in a fixed file-scope device array of
  * ACCFFT_AMD_NOALLOC_MAX_BYTES per GPU instead of the plan's workspace;
- * plans whose spectrum is larger get ACCFFT_AMD_ERR_SCRATCH.  The array is
+ * plans whose spectrum is larger get ACCFFT_AMD_ERR_SCRATCH, plans with an
+ * axis longer than one LDS tile (whose passes need workspaces of their
+ * own) ACCFFT_AMD_ERR_UNSUPPORTED.  The array is
  * shared by every call on a GPU (and keeps the last call's contents), so
  * calls that overlap on one GPU -- on different streams or host threads --
  * race; order them on one stream.  The current device must be the plan's.

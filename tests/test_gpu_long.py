@@ -1,9 +1,11 @@
 """Axes longer than one LDS tile (FP32 > 10240 points, FP64 > 5120 points
 that are not powers of two up to 4096), which cuFFT plans accept
 (src/accfft_gpu.cpp:173-253, any n) and which run here as one global radix
-pass per stage of the mixed-radix schedule: r2c, c2r and c2c with the long
-axis in each position, a prime length (one O(p) pass per output), and the
-spectral operators, against the FP64 oracle."""
+pass per stage of the mixed-radix schedule, or, for lengths with a prime
+factor above 61, as Bluestein's chirp convolution over power-of-two FFTs:
+r2c, c2r and c2c with the long axis in each position, prime and
+large-prime-factor lengths, and the spectral operators, against the FP64
+oracle."""
 import numpy as np
 import pytest
 
@@ -26,6 +28,8 @@ CASES = [
     ((3, 2, 6000), "f64"), ((2, 5153, 3), "f64"), ((5153, 2, 2), "f64"),
     ((2, 3, 16384), "f32"), ((16384, 2, 3), "f32"), ((2, 16384, 2), "f32"),
     ((3, 2, 12000), "f32"), ((2, 3, 10243), "f32"),
+    # Bluestein: primes and a large prime factor (8128 = 64 * 127)
+    ((2, 2, 10007), "f64"), ((8128, 2, 3), "f64"), ((3, 12289, 2), "f32"), ((20011, 2, 2), "f32"),
 ]
 
 
@@ -67,7 +71,7 @@ def test_long_r2c_c2r_c2c(dev, orc, shape, prec):
     pi.destroy()
 
 
-@pytest.mark.parametrize("shape", [(4, 3, 8192), (8192, 4, 3), (3, 6000, 4)])
+@pytest.mark.parametrize("shape", [(4, 3, 8192), (8192, 4, 3), (3, 6000, 4), (4, 5158, 2)])
 def test_long_operators_fp64(dev, orc, shape):
     """laplace, inverse laplace, grad (the 3-D form on long axes) and
     divergence against the oracle's operators."""

@@ -64,3 +64,8 @@ def test_noalloc_rejects_oversize(dev):
     assert st == 8, st  # ACCFFT_AMD_ERR_SCRATCH, before any launch
     assert b"scratch" in L.accfft_amd_status_string(st)
     p.destroy()
+    # an axis longer than one LDS tile needs the plan's own workspaces
+    q = af.Plan("r2c", (2, 3, 16384), None, None, af.F32, af.ACCFFT_ESTIMATE)
+    st = _noalloc(L)(q.handle, u.data_ptr(), u.data_ptr(), None)
+    assert st == 3, st  # ACCFFT_AMD_ERR_UNSUPPORTED, before any launch
+    q.destroy()
