@@ -36,7 +36,7 @@ enum {
   ACCFFT_AMD_OK = 0,
   ACCFFT_AMD_ERR_INVALID = 1,     /* bad argument (size <= 0, bad axis, ...) */
   ACCFFT_AMD_ERR_OVERFLOW = 2,    /* a size does not fit the target type */
-  ACCFFT_AMD_ERR_UNSUPPORTED = 3, /* length too large for one LDS tile */
+  ACCFFT_AMD_ERR_UNSUPPORTED = 3, /* length, layout or mode this call does not support */
   ACCFFT_AMD_ERR_NOT_BAKED = 4,   /* plan of the wrong kind for this call */
   ACCFFT_AMD_ERR_HIP = 5,         /* a HIP runtime call failed */
   ACCFFT_AMD_ERR_ALLOC = 6,       /* device allocation failed */
