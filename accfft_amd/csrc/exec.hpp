@@ -815,6 +815,11 @@ struct Ops {
     };
     return even(p->p0) && even(p->p1);
   }
+  // column blocks of width w (< 2^16: the z lengths of one LDS tile)
+  static void set_zcol(FftArgs<T>& a, int64_t w) {
+    a.zcol = w;
+    a.zmag = w > 1 ? (uint32_t)((((uint64_t)1 << 32) + (uint64_t)w - 1) / (uint64_t)w) : 0u;
+  }
   static int r2c_pencil_fused(accfft_amd_plan* p, const void* in, void* out, const Xchg& t1,
                               const Xchg& t2) {
     const int64_t N1 = p->n[1], N2 = p->n[2], Nh = N2 / 2 + 1;
@@ -824,7 +829,7 @@ struct Ops {
     int st;
     // z r2c into the row exchange's send blocks [q][n0 n1][zn_q]
     auto az = zargs(in, p->ws[0], n0 * n1, N2, Nh);
-    if (P1 > 1) az.zcol = p->zn[0];
+    if (P1 > 1) set_zcol(az, p->zn[0]);
     if ((st = run_pass<T>(p, "z", false, M_R2C, -1, az, tab(p, 2), 0))) return st;
     const void* yin = p->ws[0];
     if (P1 > 1 && (st = pencil_exchange(p, 0, t1, p->ws[0], p->ws[1], -1, &yin))) return st;
@@ -863,7 +868,7 @@ struct Ops {
     if (P1 > 1 && (st = pencil_exchange(p, 0, t1, yout, p->ws[0], +1, &zin))) return st;
     // z c2r from the received column blocks [q][n0 n1][zn_q]
     auto az = zargs(zin, out, n0 * n1, Nh, N2);
-    if (P1 > 1) az.zcol = p->zn[0];
+    if (P1 > 1) set_zcol(az, p->zn[0]);
     return run_pass<T>(p, "z", false, M_C2R, +1, az, tab(p, 2), 5);
   }
   static int r2c_pencil(accfft_amd_plan* p, const void* in, void* out) {

@@ -57,8 +57,10 @@ struct FftArgs {
   // zcol (the last one cut at n/2+1), block q of every line stored together
   // at q * zcol * nlines as [nlines][width of block q]; 0 = plain rows.
   // This is the pack / unpack of a pencil share's row exchange fused into
-  // the z pass.
+  // the z pass.  zmag = ceil(2^32 / zcol): k / zcol = (k zmag) >> 32 exactly
+  // for k, zcol < 2^16 (zcol = 1: unused).
   int64_t zcol;
+  uint32_t zmag;
 };
 
 // Element k of half-spectrum line g of a z pass (see FftArgs::zcol).
@@ -66,8 +68,8 @@ template <typename T>
 __device__ __forceinline__ int64_t zcol_off(const FftArgs<T>& a, int64_t g, int64_t dist, int k,
                                             int nh) {
   if (a.zcol == 0) return g * dist + k;
-  const int zc = (int)a.zcol;  // < n / 2 + 1: 32-bit division
-  const int s = k / zc * zc;
+  const int zc = (int)a.zcol;
+  const int s = zc == 1 ? k : (int)__umulhi((unsigned)k, a.zmag) * zc;
   const int w = (nh - s) < zc ? (nh - s) : zc;
   return (int64_t)s * a.nlines + g * w + (k - s);
 }
