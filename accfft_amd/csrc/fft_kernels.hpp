@@ -1268,7 +1268,8 @@ __global__ void __launch_bounds__((R / 2 * sched_minnb<NZ, 0>()))
   cpx<T> w[R];
 #pragma unroll
   for (int k1 = 0; k1 < R; ++k1) w[k1] = twy[n2 * k1];
-  cpx<T>* dst = reinterpret_cast<cpx<T>*>(a.out) + x * a.Dout + n2 * (IL ? R : M) * a.odist;
+  // row k1 + R n2 (IL) or k1 M + n2
+  cpx<T>* dst = reinterpret_cast<cpx<T>*>(a.out) + x * a.Dout + n2 * (IL ? R : 1) * a.odist;
   const int64_t rstep = (IL ? 1 : M) * a.odist;
   const T h = (T)0.5;
   for (int k = t; k < NH; k += TPB) {
@@ -1302,7 +1303,8 @@ __global__ void __launch_bounds__((R / 2 * sched_minnb<NZ, 0>()))
   const int t = threadIdx.x;
   const int64_t blk = xcd_tile(blockIdx.x, gridDim.x, (unsigned)a.xgroup);
   const int64_t M = a.C, x = blk / M, n2 = blk % M;
-  const cpx<T>* src = reinterpret_cast<const cpx<T>*>(a.in) + x * a.Din + n2 * (IL ? R : M) * a.idist;
+  // row k1 + R n2 (IL) or k1 M + n2
+  const cpx<T>* src = reinterpret_cast<const cpx<T>*>(a.in) + x * a.Din + n2 * (IL ? R : 1) * a.idist;
   const int64_t rstep = (IL ? 1 : M) * a.idist;
   cpx<T> w[R];
 #pragma unroll

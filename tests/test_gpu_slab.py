@@ -143,11 +143,6 @@ def test_slab_full_share_fp32(dev):
     plan.destroy()
 
 
-FUSED_OFF = ("the slab share's fused z pass with the first y level is switched off: a GPU run "
-             "was lost while these tests ran (cause not found); kept off the GPU for now")
-
-
-@pytest.mark.skip(reason=FUSED_OFF)
 @pytest.mark.parametrize("N,P", [((16, 2048, 2048), 8), ((2, 2048, 2048), 1), ((64, 2048, 2048), 64)])
 def test_slab_fused_first_y_level_fp32(dev, orc, N, P):
     """2048-point y and z axes in FP32: the z pass carries the first radix-8
@@ -176,7 +171,6 @@ def test_slab_fused_first_y_level_fp32(dev, orc, N, P):
     plan.destroy()
 
 
-@pytest.mark.skip(reason=FUSED_OFF)
 def test_slab_fused_exchange_hook_fp32(dev):
     """The fused 2048-point slab path through an exchange hook that copies
     the blocks: same bits as the identity exchange, forward and inverse."""
