@@ -147,8 +147,10 @@ int accfft_amd_plan_set_exchange(accfft_amd_plan* plan, accfft_amd_exchange_fn f
  * p1 in equal power-of-two blocks (or a communicator has one rank) the
  * packs and unpacks are the z and y passes' own stores and loads; other
  * splits run them as separate copies.  p0, p1 <= 64; out of place; no
- * split-precision flags; lines must fit one LDS tile (FP32 <= 10240, FP64
- * <= 5120 points, powers of two up to 4096 in FP64); full xyz only. */
+ * split-precision flags (ACCFFT_MEASURE is accepted: the passes use the
+ * built-in tile rule, or accfft_amd_plan_set_tuning); lines must fit one
+ * LDS tile (FP32 <= 10240, FP64 <= 5120 points, powers of two up to 4096
+ * in FP64); full xyz only. */
 int accfft_amd_plan_dft_3d_r2c_pencil(const int n[3], int p0, int p1, int c0, int c1,
                                       void* data, void* data_out, int fp, unsigned flags,
                                       accfft_amd_plan** plan);
