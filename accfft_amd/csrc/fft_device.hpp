@@ -23,6 +23,30 @@ struct alignas(2 * sizeof(T)) cpx {
   T x, y;
 };
 
+// Pointer into device (global) memory.  Loops that hide a running pointer
+// from the compiler (asm volatile "+v", so that it does not keep every row
+// address in registers) type it as global: a generic pointer compiles to
+// flat loads and stores, which also count on the LDS counter, so every
+// wait for an LDS result would wait for them as well.  (The host pass,
+// which only parses device code, sees a plain pointer.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+#else
+template <typename T>
+using gptr = T*;
+#endif
+template <typename T>
+__device__ __forceinline__ gptr<T> to_global(T* p) {
+  return (gptr<T>)p;
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> ld_global(gptr<const cpx<T>> p) {
+  cpx<T> r;
+  r = *p;
+  return r;
+}
+
 template <typename T>
 __device__ __forceinline__ cpx<T> cadd(cpx<T> a, cpx<T> b) { return {a.x + b.x, a.y + b.y}; }
 template <typename T>

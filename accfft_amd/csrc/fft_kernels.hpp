@@ -733,11 +733,11 @@ struct S4Tile {
       } else {
         // running pointer, opaque to the compiler: the R1 row offsets are
         // not precomputed into registers
-        const cpx<T>* p = src + (int64_t)(r0 + d * RS1) * a.Sin;
+        gptr<const cpx<T>> p = to_global(src + (int64_t)(r0 + d * RS1) * a.Sin);
         const int64_t step = (int64_t)M * a.Sin;
 #pragma unroll
         for (int n1 = 0; n1 < R1; ++n1) {
-          v[d][n1] = ok ? *p : cpx<T>{(T)0, (T)0};
+          v[d][n1] = ok ? ld_global(p) : cpx<T>{(T)0, (T)0};
           p += step;
           asm volatile("" : "+v"(p));
         }
@@ -864,11 +864,11 @@ struct S4Tile {
       } else {
         // running pointers (rows kc0 + kk + R1 * k2), opaque to the
         // compiler so that the row addresses are not all kept in registers
-        cpx<T>* p0 = dst + (int64_t)(kc0 + R1 * r0) * a.Sout;
+        gptr<cpx<T>> p0 = to_global(dst + (int64_t)(kc0 + R1 * r0) * a.Sout);
         const int64_t step = (int64_t)R1 * RS1 * a.Sout;
 #pragma unroll
         for (int kk = 0; kk < KC; ++kk) {
-          cpx<T>* p = p0;
+          gptr<cpx<T>> p = p0;
 #pragma unroll
           for (int k2 = r0; k2 < M; k2 += RS1) {
             *p = buf[(kk * M + k2) * W + c];
@@ -1046,11 +1046,11 @@ __global__ void __launch_bounds__((S4FCfg<T, N>::TPB), (MODE == M_DERIV ? S4FCfg
     const cpx<T>* src = reinterpret_cast<const cpx<T>*>(a.in) + ib;
 #pragma unroll
     for (int d = 0; d < D1; ++d) {
-      const cpx<T>* p = src + (int64_t)(r0 + d * RS1) * a.Sin;
+      gptr<const cpx<T>> p = to_global(src + (int64_t)(r0 + d * RS1) * a.Sin);
       const int64_t step = (int64_t)M * a.Sin;
 #pragma unroll
       for (int n1 = 0; n1 < R1; ++n1) {
-        v[d][n1] = ok ? *p : cpx<T>{(T)0, (T)0};
+        v[d][n1] = ok ? ld_global(p) : cpx<T>{(T)0, (T)0};
         p += step;
         asm volatile("" : "+v"(p));
       }
@@ -1182,15 +1182,15 @@ __global__ void __launch_bounds__((S4FCfg<T, N>::TPB), (MODE == M_DERIV ? S4FCfg
     for (int k1 = 1; k1 < R1; ++k1) w[k1] = cmulc(w[k1], tw1[n2 * k1]);
     dft<T, R1, +1>(w);
     if (ok) {
-      cpx<T>* p = dst + (int64_t)n2 * a.Sout;
+      gptr<cpx<T>> p = to_global(dst + (int64_t)n2 * a.Sout);
       const int64_t step = (int64_t)M * a.Sout;
       if (MODE == M_DERIV && a.accumulate) {  // uniform: one loop per case
         // the accumulator loads all in flight, then the stores
         cpx<T> o[R1];
-        const cpx<T>* pl = p;
+        gptr<const cpx<T>> pl = p;
 #pragma unroll
         for (int n1 = 0; n1 < R1; ++n1) {
-          o[n1] = *pl;
+          o[n1] = ld_global(pl);
           pl += step;
           asm volatile("" : "+v"(pl));
         }
