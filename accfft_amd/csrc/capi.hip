@@ -141,6 +141,30 @@ int maybe_tune(accfft_amd_plan* p) {
   return p->fp == ACCFFT_AMD_F32 ? afft_tune_f32(p) : afft_tune_f64(p);
 }
 
+// The tile queue of the plan's persistent strided passes (sfft4_kernel):
+// zeroed once here, left zeroed by every launch that uses it.
+int make_queue(accfft_amd_plan* p) {
+  void* q = nullptr;
+  const size_t bytes = afft::QUEUE_WORDS * sizeof(unsigned);
+  if (hipMalloc(&q, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return ACCFFT_AMD_ERR_ALLOC;
+  }
+  p->dev_mem.push_back(q);
+  // a blocking copy (not a memset, which may run after the plan's first
+  // launch on a non-blocking stream)
+  const std::vector<unsigned> zero(afft::QUEUE_WORDS, 0u);
+  if (hipMemcpy(q, zero.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) return ACCFFT_AMD_ERR_HIP;
+  unsigned* const u = (unsigned*)q;
+  for (int ax = 0; ax < 3; ++ax) {
+    p->tf[ax].queue = u;
+    p->td[ax].queue = u;
+  }
+  p->tf_ysub.queue = u;
+  p->td_ysub.queue = u;
+  return ACCFFT_AMD_OK;
+}
+
 int make_plan(int kind, const int n[3], void* data, void* data_out, int fp, unsigned flags,
               accfft_amd_plan** out) {
   if (!out) return ACCFFT_AMD_ERR_NULL;
@@ -178,6 +202,7 @@ int make_plan(int kind, const int n[3], void* data, void* data_out, int fp, unsi
     return ACCFFT_AMD_ERR_HIP;
   }
   int st = fp == ACCFFT_AMD_F32 ? afft_tables_f32(p) : afft_tables_f64(p);
+  if (!st) st = make_queue(p);
   if (!st && kind < 3) st = maybe_tune(p);
   if (st) {
     free_plan(p);

@@ -61,7 +61,17 @@ struct FftArgs {
   // for k, zcol < 2^16 (zcol = 1: unused).
   int64_t zcol;
   uint32_t zmag;
+  // persistent strided tiles only: the plan's tile queue (QUEUE_WORDS
+  // counters, zero between launches; nullptr = a fixed tile sequence per
+  // workgroup).  Set by the launcher.
+  unsigned* queue;
 };
+
+// Tile queue of the persistent strided tiles: one counter per XCD class of
+// workgroups (blockIdx & 7) at word 32 x, the count of finished workgroups
+// at word QUEUE_DONE; 128 bytes apart.
+constexpr int QUEUE_DONE = 8 * 32;
+constexpr int QUEUE_WORDS = QUEUE_DONE + 32;
 
 // Element k of half-spectrum line g of a z pass (see FftArgs::zcol).
 template <typename T>
@@ -944,12 +954,22 @@ __global__ void __launch_bounds__((S4Cfg<T, N, V>::TPB), (S4Cfg<T, N, V>::WPE))
     __syncthreads();
     K::run(v, a, tile, c, r0, buf, tw1, tw2);
   } else {
-    // Persistent: block b takes the virtual blocks b, b + grid, ... of a
-    // launch of one block per tile (grid a multiple of 8, so each keeps its
-    // XCD).  The whole first level goes to LDS in one chunk, after which
-    // the registers take the next tile's loads while the second level and
-    // the stores of the current tile run.
+    // Persistent: block b starts with virtual block b of a launch of one
+    // block per tile (grid G a multiple of 8, so each keeps its XCD) and
+    // takes its next ones from the plan's queue: the k-th take of class
+    // x = b & 7 is virtual block G + 8 k + x, so the tiles the blocks of an
+    // XCD work on at any time stay neighbours (the chunk map of tile_of
+    // gives them adjacent columns, and their row segments meet in the same
+    // DRAM pages), where a fixed sequence b, b + G, ... drifts apart (an
+    // access-pattern copy of the 2048-row tiles: 3.6 TB/s fixed, 4.7 TB/s
+    // queued).  Without a queue the fixed sequence is used.  The whole
+    // first level goes to LDS in one chunk, after which the registers take
+    // the next tile's loads while the second level and the stores of the
+    // current tile run.
     static_assert(K::KC == K::R1, "persistent tiles need the whole first level in LDS");
+    __shared__ int64_t next_vb;
+    unsigned* const q = a.queue;
+    const unsigned xc = blockIdx.x & 7u;
     const int64_t ntiles = (a.nlines + K::W - 1) / K::W;
     int64_t vb = blockIdx.x;
     int64_t tile = K::tile_of(vb, ntiles, a);
@@ -957,10 +977,16 @@ __global__ void __launch_bounds__((S4Cfg<T, N, V>::TPB), (S4Cfg<T, N, V>::WPE))
     K::tw_store(tv1, tv2, tw1, tw2, t);
     __syncthreads();
     for (;;) {
+      // the take is issued before the first level so that its latency
+      // overlaps it; next_vb is read after the barrier below and not
+      // written again before second()'s closing barrier
+      unsigned k = 0;
+      if (q != nullptr && t == 0) k = atomicAdd(&q[xc * 32], 1u);
       K::first(v, t, tw1);
       K::put(v, 0, c, r0, buf);
+      if (q != nullptr && t == 0) next_vb = (int64_t)gridDim.x + 8 * (int64_t)k + xc;
       __syncthreads();
-      const int64_t nvb = vb + gridDim.x;
+      const int64_t nvb = q != nullptr ? next_vb : vb + gridDim.x;
       const bool more = nvb < ntiles;
       const int64_t ntile = more ? K::tile_of(nvb, ntiles, a) : 0;
       if (more) K::load(v, a, ntile, c, r0);
@@ -968,6 +994,15 @@ __global__ void __launch_bounds__((S4Cfg<T, N, V>::TPB), (S4Cfg<T, N, V>::WPE))
       if (!more) break;
       vb = nvb;
       tile = ntile;
+    }
+    // Every block has made its last take (its result ended the loop) before
+    // it counts itself done; the last one to finish zeroes the queue for
+    // the next launch.
+    if (q != nullptr && t == 0) {
+      if (atomicAdd(&q[QUEUE_DONE], 1u) == gridDim.x - 1) {
+        for (int i = 0; i < 8; ++i) atomicExch(&q[i * 32], 0u);
+        atomicExch(&q[QUEUE_DONE], 0u);
+      }
     }
   }
 }

@@ -39,6 +39,9 @@ struct AxisTab {
   int s4v = -1;
   // tiles per XCD chunk picked by the tuning (xcd_tile's g); -1 the rule
   int xg = -1;
+  // the plan's tile queue of the persistent strided tiles (QUEUE_WORDS
+  // zeroed counters, see sfft4_kernel); nullptr = fixed tile sequences
+  unsigned* queue = nullptr;
   // lines longer than one LDS tile: global radix passes (fft_long.hpp)
   bool longl = false;
   // long lines with a prime factor above 61: Bluestein's algorithm over
@@ -202,9 +205,11 @@ inline bool four_step(const AxisTab<T>& tab) {
 // Variant of the two-level configuration for length n (S4Cfg<T, n, V>);
 // blk: rows in exchange blocks (the slab plan's y pass); tuned: the plan's
 // MEASURE choice (-1 none).  ACCFFT_AMD_S4V overrides; the built-in rule is
-// the persistent 2048 tiles for plain rows (measured 8 % faster on the 2048
-// x passes of the 256 x 2048 x 2048 slab) and variant 0 otherwise.
-inline int four_step_rule(int n, bool blk) { return (n == 2048 && !blk) ? 2 : 0; }
+// the persistent tiles (variant 2, tile queue) for 1024- and 2048-point
+// plain rows and variant 0 otherwise (the 2048 x pass of the 256 x 2048 x
+// 2048 slab share: forward 5.83 ms against 6.43 ms with variant 0; 1024^3
+// round trip 9.96 against 10.14 ms).
+inline int four_step_rule(int n, bool blk) { return ((n == 2048 || n == 1024) && !blk) ? 2 : 0; }
 inline int four_step_variant(int n, bool blk, int tuned = -1) {
   static const std::vector<std::pair<int, int>> tab = [] {
     std::vector<std::pair<int, int>> t;
@@ -270,10 +275,14 @@ hipError_t launch_s4v(int sign, FftArgs<T>& a, const AxisTab<T>& tab, hipStream_
     a.tw = tab.tw_full;
     a.lpw = CF::W;
     int64_t grid = tiles;
+    a.queue = nullptr;
     if constexpr (CF::PB > 0) {
-      // persistent: PB workgroups per CU, a multiple of 8 (one per XCD)
+      // persistent: PB workgroups per CU, a multiple of 8 (one per XCD);
+      // the queue hands out the tiles beyond the first grid's worth (every
+      // class of blockIdx & 7 must have workgroups)
       const int64_t cap = (int64_t)device_cus() * CF::PB;
       if (grid > cap) grid = cap >= 8 ? cap / 8 * 8 : cap;
+      if (grid < tiles && grid % 8 == 0) a.queue = tab.queue;
     }
     dim3 g((unsigned)grid), b(CF::TPB);
     const bool blk = a.ishift != 0 || a.oshift != 0;

@@ -1,0 +1,66 @@
+"""Persistent strided tiles with the plan's tile queue.
+
+The persistent two-level configurations (2048 v2/v3, 1024 v2/v3, 512 v3)
+start one workgroup per tile slot and hand out the remaining tiles from a
+per-plan queue of device counters that the last workgroup of each launch
+zeroes again.  On shapes with more x-pass tiles than workgroups the queue
+is in use; repeated executions of one plan must each match the FP64 oracle
+(src/accfft_gpu.cpp:444-623, rel-L2 <= 1e-5 FP32) and agree bit for bit
+(every tile runs the same arithmetic whichever workgroup takes it).
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2, rng
+
+pytestmark = pytest.mark.gpu
+
+# (shape, variant of the x axis): x-pass tiles of 8 or 16 columns well
+# beyond one per CU (256) or two per CU (PB = 2)
+CASES = [
+    ((2048, 40, 254), 2),
+    ((2048, 40, 254), 3),
+    ((1024, 64, 254), 2),
+    ((1024, 64, 254), 3),
+    ((512, 128, 254), 3),
+]
+
+
+@pytest.mark.parametrize("shape,variant", CASES)
+def test_repeated_executions_match_oracle(dev, orc, shape, variant):
+    import torch
+    import accfft_amd as af
+    print(f"tile queue shape={shape} variant={variant}", flush=True)
+    x = rng(sum(shape) + variant).standard_normal(shape)
+    a = torch.tensor(x, dtype=torch.float32, device=dev)
+    h = torch.empty((shape[0], shape[1], shape[2] // 2 + 1), dtype=torch.complex64, device=dev)
+    back = torch.empty_like(a)
+    plan = af.Plan("r2c", shape, a, h, af.F32, af.ACCFFT_ESTIMATE)
+    plan.set_tuning(0, variant)
+    ref = orc.r2c(x)
+    first = None
+    for rep in range(3):
+        h.zero_()
+        plan.execute_r2c(a, h)
+        torch.cuda.synchronize()
+        got = h.cpu().numpy()
+        assert rel_l2(got, ref) < 1e-5, rep
+        if first is None:
+            first = got
+        else:
+            assert np.array_equal(got, first), rep
+    refb = orc.c2r(ref, list(shape))
+    firstb = None
+    for rep in range(2):
+        # c2r overwrites its input (as the reference's does): a fresh copy
+        hr = torch.tensor(ref, dtype=torch.complex64, device=dev)
+        back.zero_()
+        plan.execute_c2r(hr, back)
+        torch.cuda.synchronize()
+        got = back.cpu().numpy()
+        assert rel_l2(got, refb) < 1e-5, rep
+        if firstb is None:
+            firstb = got
+        else:
+            assert np.array_equal(got, firstb), rep
+    plan.destroy()
