@@ -692,7 +692,9 @@ AFFT_S4CFG(float, 512, 2, 16, 16, 512, 8, 1, 0, 1, 0)
 AFFT_S4CFG(float, 512, 3, 16, 16, 512, 16, 0, 0, 4, 2)
 AFFT_S4CFG(float, 1024, 0, 16, 8, 512, 8, 0, 0, 1, 0)
 AFFT_S4CFG(float, 1024, 1, 16, 16, 1024, 8, 0, 0, 1, 0)
-AFFT_S4CFG(float, 1024, 2, 16, 16, 1024, 16, 0, 0, 4, 1)
+// (1024 v2: power twiddles keep it within 128 VGPRs; with the full table
+// two spilled registers were reloaded per tile, +6 % HBM reads)
+AFFT_S4CFG(float, 1024, 2, 16, 16, 1024, 16, 1, 0, 4, 1)
 AFFT_S4CFG(float, 1024, 3, 16, 8, 512, 16, 0, 0, 4, 2)
 AFFT_S4CFG(float, 2048, 0, 16, 8, 1024, 16, 0, 0, 1, 0)
 AFFT_S4CFG(float, 2048, 1, 16, 4, 512, 16, 0, 0, 1, 0)
@@ -735,10 +737,16 @@ struct S4Tile {
 #pragma unroll
     for (int d = 0; d < D1; ++d) {
       if constexpr (BLK) {
+        // rows r0 + d RS1 + M n1 in exchange blocks; the row counter is
+        // opaque to the compiler, so the R1 offsets are formed one at a
+        // time instead of all being kept in registers
+        const gptr<const cpx<T>> g = to_global(src);
+        int row = r0 + d * RS1;
 #pragma unroll
         for (int n1 = 0; n1 < R1; ++n1) {
-          const int row = r0 + d * RS1 + M * n1;
-          v[d][n1] = ok ? src[row_off(row, a.Sin, a.ishift, a.Bin)] : cpx<T>{(T)0, (T)0};
+          v[d][n1] = ok ? ld_global(g + row_off(row, a.Sin, a.ishift, a.Bin)) : cpx<T>{(T)0, (T)0};
+          row += M;
+          asm volatile("" : "+v"(row));
         }
       } else {
         // running pointer, opaque to the compiler: the R1 row offsets are
@@ -864,13 +872,21 @@ struct S4Tile {
     if (ok) {
       cpx<T>* dst = reinterpret_cast<cpx<T>*>(a.out) + ob;
       if constexpr (BLK) {
+        // rows kc0 + kk + R1 k2 in exchange blocks, one opaque row counter
+        const gptr<cpx<T>> g = to_global(dst);
+        int row0 = kc0 + R1 * r0;
 #pragma unroll
-        for (int kk = 0; kk < KC; ++kk)
+        for (int kk = 0; kk < KC; ++kk) {
+          int row = row0;
 #pragma unroll
           for (int k2 = r0; k2 < M; k2 += RS1) {
-            const int row = kc0 + kk + R1 * k2;
-            dst[row_off(row, a.Sout, a.oshift, a.Bout)] = buf[(kk * M + k2) * W + c];
+            g[row_off(row, a.Sout, a.oshift, a.Bout)] = buf[(kk * M + k2) * W + c];
+            row += R1 * RS1;
+            asm volatile("" : "+v"(row));
           }
+          row0 += 1;
+          asm volatile("" : "+v"(row0));
+        }
       } else {
         // running pointers (rows kc0 + kk + R1 * k2), opaque to the
         // compiler so that the row addresses are not all kept in registers

@@ -205,11 +205,13 @@ inline bool four_step(const AxisTab<T>& tab) {
 // Variant of the two-level configuration for length n (S4Cfg<T, n, V>);
 // blk: rows in exchange blocks (the slab plan's y pass); tuned: the plan's
 // MEASURE choice (-1 none).  ACCFFT_AMD_S4V overrides; the built-in rule is
-// the persistent tiles (variant 2, tile queue) for 1024- and 2048-point
-// plain rows and variant 0 otherwise (the 2048 x pass of the 256 x 2048 x
-// 2048 slab share: forward 5.83 ms against 6.43 ms with variant 0; 1024^3
-// round trip 9.96 against 10.14 ms).
-inline int four_step_rule(int n, bool blk) { return ((n == 2048 || n == 1024) && !blk) ? 2 : 0; }
+// the persistent tiles (variant 2, tile queue) for 2048-point rows, plain
+// or in exchange blocks, and 1024-point plain rows, variant 0 otherwise
+// (the 2048 x pass of the 256 x 2048 x 2048 slab share: forward 5.84 ms
+// against 6.47 ms with variant 0; the 2048 y pass of a 2048^3 rank on a
+// 4 x 2 pencil grid: forward 6.50 against 6.81 ms; 1024^3 round trip 9.78
+// against 10.11 ms).
+inline int four_step_rule(int n, bool blk) { return (n == 2048 || (n == 1024 && !blk)) ? 2 : 0; }
 inline int four_step_variant(int n, bool blk, int tuned = -1) {
   static const std::vector<std::pair<int, int>> tab = [] {
     std::vector<std::pair<int, int>> t;
@@ -282,7 +284,11 @@ hipError_t launch_s4v(int sign, FftArgs<T>& a, const AxisTab<T>& tab, hipStream_
       // class of blockIdx & 7 must have workgroups)
       const int64_t cap = (int64_t)device_cus() * CF::PB;
       if (grid > cap) grid = cap >= 8 ? cap / 8 * 8 : cap;
-      if (grid < tiles && grid % 8 == 0) a.queue = tab.queue;
+      static const bool queued = [] {
+        const char* e = sweep_env("ACCFFT_AMD_QUEUE");  // 0: fixed sequences (sweep builds)
+        return !e || atoi(e) != 0;
+      }();
+      if (queued && grid < tiles && grid % 8 == 0) a.queue = tab.queue;
     }
     dim3 g((unsigned)grid), b(CF::TPB);
     const bool blk = a.ishift != 0 || a.oshift != 0;
