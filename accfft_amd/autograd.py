@@ -193,11 +193,17 @@ _CTX: Dict[tuple, Spectral] = {}
 
 
 def context(n, dtype, device) -> Spectral:
-    """The cached Spectral for (shape, dtype, device)."""
-    key = (tuple(int(v) for v in n), dtype, torch.device(device))
+    """The cached Spectral for (shape, dtype, device, current stream).
+
+    One plan per stream: a plan's calls share its workspaces, and the plan
+    orders calls that arrive on different streams one after the other, so
+    work on two streams gets plans of its own to overlap."""
+    dev = torch.device(device)
+    stream = torch.cuda.current_stream(dev).cuda_stream if dev.type == "cuda" else 0
+    key = (tuple(int(v) for v in n), dtype, dev, stream)
     s = _CTX.get(key)
     if s is None:
-        s = Spectral(key[0], dtype, key[2])
+        s = Spectral(key[0], dtype, dev)
         _CTX[key] = s
     return s
 
