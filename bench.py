@@ -587,8 +587,6 @@ def cpu_baseline(n, full=False):
     sizes = [64, 256, 512] + ([1024] if full else [])
     for m in sizes:
         for dt, nm in ((torch.float32, "fp32"), (torch.float64, "fp64")):
-            if m == 1024 and dt == torch.float64:
-                continue
             a = torch.from_numpy(rng.standard_normal((m, m, m))).to(dt)
             t = _median_ms(lambda: torch.fft.irfftn(torch.fft.rfftn(a), s=a.shape))
             lib[f"{m}^3_{nm}_ms"] = round(t, 3)
