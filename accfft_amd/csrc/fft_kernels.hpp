@@ -1266,6 +1266,14 @@ __global__ void __launch_bounds__((S4FCfg<T, N>::TPB), (MODE == M_DERIV ? S4FCfg
 // ====================================================================
 // z pass fused with the first level of a two-level y transform (slab share)
 // ====================================================================
+// LDS tiles of the fused z kernels padded by one element in 16
+// (lds_pad16): the Stockham stage writes of stride R, which otherwise fall
+// on the same banks, spread out.  Only where the padding leaves the
+// occupancy as it is or nearly so (the larger tiles fill the LDS with two
+// or four blocks per CU unpadded).
+template <int NZ>
+constexpr int zy_pad() { return NZ <= 512 ? 1 : 0; }
+
 // N1 = R * M along y.  A block owns the rows y = n2 + M n1 (n1 < R) of one
 // x-plane: the R real z lines as R/2 packed complex FFTs of length NZ in
 // LDS, then, per half-spectrum column k, the R row spectra, their R-point
@@ -1293,7 +1301,9 @@ __global__ void __launch_bounds__((R / 2 * sched_minnb<NZ, 0>()))
     zy_r2c_kernel(FftArgs<T> a, const cpx<T>* __restrict__ twy) {
   constexpr int NT = sched_minnb<NZ, 0>(), LINES = R / 2, TPB = LINES * NT;
   constexpr int NH = NZ / 2 + 1, EP = LINES * NZ / TPB, TWN = sched_twn<NZ, 0>();
-  __shared__ cpx<T> buf[LINES * NZ];
+  constexpr int ZP = zy_pad<NZ>();
+  auto P = [](int e) { return ZP ? lds_pad16(e) : e; };
+  __shared__ cpx<T> buf[LINES * NZ + (ZP ? LINES * NZ / 16 : 0)];
   __shared__ cpx<T> s_tw[TWN];
   const int t = threadIdx.x;
   const int64_t blk = xcd_tile(blockIdx.x, gridDim.x, (unsigned)a.xgroup);
@@ -1309,11 +1319,11 @@ __global__ void __launch_bounds__((R / 2 * sched_minnb<NZ, 0>()))
   }
   stage_twiddles<T, TWN, TPB>(s_tw, a.tw, t);
 #pragma unroll
-  for (int j = 0; j < EP; ++j) buf[t + j * TPB] = v[j];
+  for (int j = 0; j < EP; ++j) buf[P(t + j * TPB)] = v[j];
   __syncthreads();
   {
     const int line = t / NT, q = t % NT;
-    LineAcc<T, 1> acc{buf + line * NZ};
+    LineAcc<T, 1, ZP> acc{buf + P(line * NZ)};
     static_fft<T, NZ, NT, -1, 1, 0, LINES>(acc, q, s_tw, buf, NZ);
   }
   cpx<T> w[R];
@@ -1327,8 +1337,8 @@ __global__ void __launch_bounds__((R / 2 * sched_minnb<NZ, 0>()))
     cpx<T> u[R];
 #pragma unroll
     for (int p = 0; p < LINES; ++p) {
-      const cpx<T> z1 = buf[p * NZ + k];
-      const cpx<T> z2 = buf[p * NZ + (k == 0 ? 0 : NZ - k)];
+      const cpx<T> z1 = buf[P(p * NZ + k)];
+      const cpx<T> z2 = buf[P(p * NZ + (k == 0 ? 0 : NZ - k))];
       u[2 * p] = cpx<T>{h * (z1.x + z2.x), h * (z1.y - z2.y)};
       u[2 * p + 1] = cpx<T>{h * (z1.y + z2.y), h * (z2.x - z1.x)};
     }
@@ -1349,7 +1359,9 @@ __global__ void __launch_bounds__((R / 2 * sched_minnb<NZ, 0>()))
     zy_c2r_kernel(FftArgs<T> a, const cpx<T>* __restrict__ twy) {
   constexpr int NT = sched_minnb<NZ, 0>(), LINES = R / 2, TPB = LINES * NT;
   constexpr int NH = NZ / 2 + 1, EP = LINES * NZ / TPB, TWN = sched_twn<NZ, 0>();
-  __shared__ cpx<T> buf[LINES * NZ];
+  constexpr int ZP = zy_pad<NZ>();
+  auto P = [](int e) { return ZP ? lds_pad16(e) : e; };
+  __shared__ cpx<T> buf[LINES * NZ + (ZP ? LINES * NZ / 16 : 0)];
   __shared__ cpx<T> s_tw[TWN];
   const int t = threadIdx.x;
   const int64_t blk = xcd_tile(blockIdx.x, gridDim.x, (unsigned)a.xgroup);
@@ -1387,15 +1399,15 @@ __global__ void __launch_bounds__((R / 2 * sched_minnb<NZ, 0>()))
         A.y = (T)0;
         B.y = (T)0;
       }
-      buf[p * NZ + k] = cpx<T>{A.x - B.y, A.y + B.x};
-      if (!edge) buf[p * NZ + NZ - k] = cpx<T>{A.x + B.y, B.x - A.y};
+      buf[P(p * NZ + k)] = cpx<T>{A.x - B.y, A.y + B.x};
+      if (!edge) buf[P(p * NZ + NZ - k)] = cpx<T>{A.x + B.y, B.x - A.y};
     }
   }
   stage_twiddles<T, TWN, TPB>(s_tw, a.tw, t);
   __syncthreads();
   {
     const int line = t / NT, q = t % NT;
-    LineAcc<T, 1> acc{buf + line * NZ};
+    LineAcc<T, 1, ZP> acc{buf + P(line * NZ)};
     static_fft<T, NZ, NT, +1, 1, 0, LINES>(acc, q, s_tw, buf, NZ);
   }
   T* dst = reinterpret_cast<T*>(a.out) + x * a.Dout + n2 * a.odist;
@@ -1403,7 +1415,7 @@ __global__ void __launch_bounds__((R / 2 * sched_minnb<NZ, 0>()))
 #pragma unroll
   for (int j = 0; j < EP; ++j) {
     const int e = t + j * TPB, p = e / NZ, i = e % NZ;
-    const cpx<T> z = buf[e];
+    const cpx<T> z = buf[P(e)];
     T* ra = dst + p * pair;
     ra[i] = z.x;
     ra[M * a.odist + i] = z.y;
