@@ -187,7 +187,7 @@ __device__ __forceinline__ cpx<T>* zrun(cpx<T>* A, cpx<T>* B, const FftArgs<T>& 
                                         const cpx<T>* twl, int lpw) {
   if constexpr (N > 0) {
     const int line = threadIdx.x / NT, q = threadIdx.x % NT;
-    LineAcc<T, 1, PAD> acc{A + line * (PAD ? N + N / 16 : N)};
+    LineAcc<T, 1, PAD> acc{A + line * (PAD == 1 ? N + N / 16 : N)};
     static_fft<T, N, NT, SIGN, 1, SPLIT, LINES>(acc, q, twl, A, N);
     return A;
   } else {
@@ -228,7 +228,9 @@ __global__ void __launch_bounds__(ZT > 256 ? ZT : 256) zfft_kernel(FftArgs<T> a)
   const int nh = n / 2 + 1;
   const int lpw = STATIC ? ZC::LPW : a.lpw;  // complex lines per block
   const int t = threadIdx.x;
-  // static tiles are stored padded (lds_pad16): conflict-free stage writes
+  // static tiles are stored padded (lds_pad16): the stage writes spread
+  // over the banks (the swizzle of lds_swz32 measured slower here: its
+  // index arithmetic costs the derivative pass a wave per SIMD)
   constexpr int ZPAD = (STATIC && SPLIT == 0) ? 1 : 0;
   auto P = [](int e) { return ZPAD ? lds_pad16(e) : e; };
 
@@ -1266,13 +1268,14 @@ __global__ void __launch_bounds__((S4FCfg<T, N>::TPB), (MODE == M_DERIV ? S4FCfg
 // ====================================================================
 // z pass fused with the first level of a two-level y transform (slab share)
 // ====================================================================
-// LDS tiles of the fused z kernels padded by one element in 16
-// (lds_pad16): the Stockham stage writes of stride R, which otherwise fall
-// on the same banks, spread out.  Only where the padding leaves the
-// occupancy as it is or nearly so (the larger tiles fill the LDS with two
-// or four blocks per CU unpadded).
+// LDS tiles of the fused z kernels: padded by one element in 16
+// (lds_pad16) up to 512 points, swizzled (lds_swz32, no extra LDS) from 1024
+// points, whose tiles fill the LDS at two or four blocks per CU: the
+// Stockham stage writes of stride R, which otherwise fall on the same
+// banks, spread out.  (At 512 points the swizzle measured 2 % slower than
+// the padding: z C2R 206 against 202 us.)
 template <int NZ>
-constexpr int zy_pad() { return NZ <= 512 ? 1 : 0; }
+constexpr int zy_pad() { return NZ <= 512 ? 1 : 2; }
 
 // N1 = R * M along y.  A block owns the rows y = n2 + M n1 (n1 < R) of one
 // x-plane: the R real z lines as R/2 packed complex FFTs of length NZ in
@@ -1302,8 +1305,8 @@ __global__ void __launch_bounds__((R / 2 * sched_minnb<NZ, 0>()))
   constexpr int NT = sched_minnb<NZ, 0>(), LINES = R / 2, TPB = LINES * NT;
   constexpr int NH = NZ / 2 + 1, EP = LINES * NZ / TPB, TWN = sched_twn<NZ, 0>();
   constexpr int ZP = zy_pad<NZ>();
-  auto P = [](int e) { return ZP ? lds_pad16(e) : e; };
-  __shared__ cpx<T> buf[LINES * NZ + (ZP ? LINES * NZ / 16 : 0)];
+  auto P = [](int e) { return ZP == 1 ? lds_pad16(e) : lds_swz_tile<NZ>(e); };
+  __shared__ cpx<T> buf[LINES * NZ + (ZP == 1 ? LINES * NZ / 16 : 0)];
   __shared__ cpx<T> s_tw[TWN];
   const int t = threadIdx.x;
   const int64_t blk = xcd_tile(blockIdx.x, gridDim.x, (unsigned)a.xgroup);
@@ -1360,8 +1363,8 @@ __global__ void __launch_bounds__((R / 2 * sched_minnb<NZ, 0>()))
   constexpr int NT = sched_minnb<NZ, 0>(), LINES = R / 2, TPB = LINES * NT;
   constexpr int NH = NZ / 2 + 1, EP = LINES * NZ / TPB, TWN = sched_twn<NZ, 0>();
   constexpr int ZP = zy_pad<NZ>();
-  auto P = [](int e) { return ZP ? lds_pad16(e) : e; };
-  __shared__ cpx<T> buf[LINES * NZ + (ZP ? LINES * NZ / 16 : 0)];
+  auto P = [](int e) { return ZP == 1 ? lds_pad16(e) : lds_swz_tile<NZ>(e); };
+  __shared__ cpx<T> buf[LINES * NZ + (ZP == 1 ? LINES * NZ / 16 : 0)];
   __shared__ cpx<T> s_tw[TWN];
   const int t = threadIdx.x;
   const int64_t blk = xcd_tile(blockIdx.x, gridDim.x, (unsigned)a.xgroup);
