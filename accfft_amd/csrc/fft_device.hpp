@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace afft {
 
 template <typename T>
@@ -322,12 +324,18 @@ struct LineAcc {
   __device__ __forceinline__ void st(int n, cpx<T> v) const { p[at(n)] = v; }
 };
 
+// No element-wise operation before a transform's first stage.
+struct NoPre {};
+
 // One in-place stage.  Thread q of NT threads on the line performs
 // butterflies j = q + NT*i.  Twiddles for stage s live at tw[TWOFF +
 // (r-1)*NS + k] = exp(-2 pi i r k / (NS R)); conjugated for SIGN = +1.
-template <typename T, int N, int R, int NS, int NT, int SIGN, int TWOFF, int ES, int PAD = 0>
+// pre(n, x): applied to element n of the line as it is loaded (a spectral
+// symbol folded into the first stage of an inverse transform).
+template <typename T, int N, int R, int NS, int NT, int SIGN, int TWOFF, int ES, int PAD = 0,
+          typename PRE = NoPre>
 __device__ __forceinline__ void sstage(const LineAcc<T, ES, PAD>& acc, int q,
-                                       const cpx<T>* __restrict__ tw) {
+                                       const cpx<T>* __restrict__ tw, const PRE& pre = PRE{}) {
   constexpr int NB = N / R;
   constexpr int PER = NB / NT;
   static_assert(NB % NT == 0, "threads per line must divide the butterfly count");
@@ -336,7 +344,10 @@ __device__ __forceinline__ void sstage(const LineAcc<T, ES, PAD>& acc, int q,
   for (int i = 0; i < PER; ++i) {
     const int j = q + NT * i;
 #pragma unroll
-    for (int r = 0; r < R; ++r) v[i][r] = acc.ld(j + r * NB);
+    for (int r = 0; r < R; ++r) {
+      v[i][r] = acc.ld(j + r * NB);
+      if constexpr (!std::is_same<PRE, NoPre>::value) v[i][r] = pre(j + r * NB, v[i][r]);
+    }
   }
   __syncthreads();
 #pragma unroll

@@ -182,13 +182,14 @@ __device__ __forceinline__ void stage_twiddles(cpx<T>* dst, const cpx<T>* __rest
 
 // -------------------------------------------------------------- engines
 // N > 0: static engine; N == 0: dynamic engine.
-template <typename T, int N, int NT, int SIGN, int SPLIT = 0, int LINES = 256 / NT, int PAD = 0>
+template <typename T, int N, int NT, int SIGN, int SPLIT = 0, int LINES = 256 / NT, int PAD = 0,
+          typename PRE = NoPre>
 __device__ __forceinline__ cpx<T>* zrun(cpx<T>* A, cpx<T>* B, const FftArgs<T>& a,
-                                        const cpx<T>* twl, int lpw) {
+                                        const cpx<T>* twl, int lpw, const PRE& pre = PRE{}) {
   if constexpr (N > 0) {
     const int line = threadIdx.x / NT, q = threadIdx.x % NT;
     LineAcc<T, 1, PAD> acc{A + line * (PAD == 1 ? N + N / 16 : N)};
-    static_fft<T, N, NT, SIGN, 1, SPLIT, LINES>(acc, q, twl, A, N);
+    static_fft<T, N, NT, SIGN, 1, SPLIT, LINES, PAD, PRE>(acc, q, twl, A, N, pre);
     return A;
   } else {
     LayZ lay{a.sc.n};
@@ -366,19 +367,30 @@ __global__ void __launch_bounds__(ZT > 256 ? ZT : 256) zfft_kernel(FftArgs<T> a)
     res = zrun<T, N, NT, +1, SPLIT, ZC::LPW, ZPAD>(bufA, bufB, a, twl, lpw);
   } else {  // M_DERIV and M_MULT (z has no MULT use; treat as DERIV-style symbol)
     res = zrun<T, N, NT, -1, SPLIT, ZC::LPW, ZPAD>(bufA, bufB, a, twl, lpw);
-    const int epm = STATIC ? ZC::LPW * NS_ / TPB : (lpw * n + TPB - 1) / TPB;
+    if constexpr (STATIC && !(split_sched(SPLIT) && SchedOf<NS_, SPLIT>::r(0) == 16)) {
+      // i w(k) scale folded into the inverse transform's first-stage loads
+      // (no LDS round trip and barrier of its own)
+      const T sc = (T)a.scale;
+      auto sym = [sc](int k, cpx<T> v) -> cpx<T> {
+        const T f = sc * (T)(int)wave_sym(k, NS_);
+        return {-f * v.y, f * v.x};
+      };
+      res = zrun<T, N, NT, +1, SPLIT, ZC::LPW, ZPAD>(res, res, a, twl, lpw, sym);
+    } else {
+      const int epm = STATIC ? ZC::LPW * NS_ / TPB : (lpw * n + TPB - 1) / TPB;
 #pragma unroll
-    for (int j = 0; j < epm; ++j) {
-      const int e = t + j * TPB;
-      if (!STATIC && e >= lpw * n) break;
-      const int i = e % n;
-      const T f = (T)(a.scale * (double)wave_sym(i, n));
-      const cpx<T> v = res[P(e)];
-      res[P(e)] = {-f * v.y, f * v.x};
+      for (int j = 0; j < epm; ++j) {
+        const int e = t + j * TPB;
+        if (!STATIC && e >= lpw * n) break;
+        const int i = e % n;
+        const T f = (T)(a.scale * (double)wave_sym(i, n));
+        const cpx<T> v = res[P(e)];
+        res[P(e)] = {-f * v.y, f * v.x};
+      }
+      __syncthreads();
+      cpx<T>* other = (res == bufA) ? bufB : bufA;
+      res = zrun<T, N, NT, +1, SPLIT, ZC::LPW, ZPAD>(res, other, a, twl, lpw);
     }
-    __syncthreads();
-    cpx<T>* other = (res == bufA) ? bufB : bufA;
-    res = zrun<T, N, NT, +1, SPLIT, ZC::LPW, ZPAD>(res, other, a, twl, lpw);
   }
 
   // ---------------- store
@@ -1140,19 +1152,21 @@ __global__ void __launch_bounds__((S4FCfg<T, N>::TPB), (MODE == M_DERIV ? S4FCfg
   __syncthreads();
   static_fft<T, M, NT, -1, W, 0, 1>(acc, q, tw2, buf, 0);
 
-  // symbol on X[k1 + R1 k2] at (k1, k2); thread t's elements all lie in
-  // its own column c (TPB is a multiple of W)
+  // symbol on X[k1 + R1 k2], folded into the first-stage loads of the
+  // inverse M-point level (no LDS round trip and barrier of its own):
+  // element n of this thread's line (k1 = kc0 + kl, column c) is
+  // X[k1 + R1 n]; the column's fixed-axis wavenumbers are this thread's.
   {
     int64_t u = 0, Z = 0;
     if constexpr (MODE == M_MULT) {
       u = col / a.Ch;
       Z = col % a.Ch;
     }
-    // one loop per symbol kind (the kind is uniform: the switch stays out
-    // of the loop).  The grad and laplace-family symbols take a fast form:
-    // the wavenumbers of the two fixed axes once per thread, the varying
-    // one per element, 32-bit integers and T arithmetic (|k|^2 sums below
-    // 2^24 are exact in FP32); the Poisson symbols use apply_sym.
+    // one inverse level per symbol kind (the kind is uniform: the switch
+    // stays out of the loads).  The grad and laplace-family symbols take a
+    // fast form: the wavenumbers of the two fixed axes once per thread, the
+    // varying one per element, 32-bit integers and T arithmetic (|k|^2 sums
+    // below 2^24 are exact in FP32); the Poisson symbols use apply_sym.
     const int Nk = (int)(a.maxis == 0 ? a.N0 : a.N1), Nf = (int)(a.maxis == 0 ? a.N1 : a.N0);
     const int N2i = (int)a.N2;
     auto nyq0 = [](int x, int n) { int w = x > n / 2 ? x - n : x; return x == n / 2 ? 0 : w; };
@@ -1160,60 +1174,56 @@ __global__ void __launch_bounds__((S4FCfg<T, N>::TPB), (MODE == M_DERIV ? S4FCfg
     const int wf0 = nyq0((int)u, Nf), wz0 = nyq0((int)Z, N2i);
     const int wf1 = plain((int)u, Nf), wz1 = plain((int)Z, N2i);
     const T sc = (T)a.scale;
-    auto sym_loop = [&](auto kind) {
+    auto sym_elem = [&](auto kind, int k, cpx<T> x) -> cpx<T> {
       constexpr int KIND = decltype(kind)::value;
-#pragma unroll
-      for (int i = 0; i < KC * D1; ++i) {
-        const int e = t + i * TPB;
-        const int k2 = (e / W) % M, k1 = kc0 + e / (M * W);
-        const int k = k1 + R1 * k2;
-        const cpx<T> x = buf[e];
-        if constexpr (MODE == M_DERIV) {
-          const T f = sc * (T)wave_sym(k, N);
-          buf[e] = cpx<T>{-f * x.y, f * x.x};
-        } else if constexpr (KIND <= SYM_GRAD_Z) {
-          // i * w_d * scale, w_d of the varying axis or a fixed one
-          const bool on_k = (KIND == SYM_GRAD_X && a.maxis == 0) || (KIND == SYM_GRAD_Y && a.maxis == 1);
-          const int w = on_k ? nyq0(k, Nk) : KIND == SYM_GRAD_Z ? wz0 : wf0;
-          const T f = sc * (T)w;
-          buf[e] = cpx<T>{-f * x.y, f * x.x};
-        } else if constexpr (KIND <= SYM_INV_LAPLACE) {
-          const int wk = nyq0(k, Nk);
-          const T wave = -(T)(wk * wk + wf0 * wf0 + wz0 * wz0);
-          T f;
-          if constexpr (KIND == SYM_LAPLACE) f = sc * wave;
-          else if constexpr (KIND == SYM_BIHARMONIC) f = (sc * wave) * wave;
-          else f = wave == (T)0 ? (T)0 : sc / wave;
-          buf[e] = cpx<T>{f * x.x, f * x.y};
-        } else if constexpr (KIND == SYM_INV_BIHARMONIC) {
-          const int wk = plain(k, Nk);
-          const T wave = (T)(wk * wk + wf1 * wf1 + wz1 * wz1);
-          const T f = wave == (T)0 ? sc : (sc / wave) / wave;
-          buf[e] = cpx<T>{f * x.x, f * x.y};
-        } else {
-          const long X = a.maxis == 0 ? (long)k : (long)u, Y = a.maxis == 0 ? (long)u : (long)k;
-          buf[e] = apply_sym<T>(KIND, X, Y, (long)Z, a.N0, a.N1, a.N2, a.scale, x);
-        }
+      if constexpr (MODE == M_DERIV) {
+        const T f = sc * (T)wave_sym(k, N);
+        return cpx<T>{-f * x.y, f * x.x};
+      } else if constexpr (KIND <= SYM_GRAD_Z) {
+        // i * w_d * scale, w_d of the varying axis or a fixed one
+        const bool on_k = (KIND == SYM_GRAD_X && a.maxis == 0) || (KIND == SYM_GRAD_Y && a.maxis == 1);
+        const int w = on_k ? nyq0(k, Nk) : KIND == SYM_GRAD_Z ? wz0 : wf0;
+        const T f = sc * (T)w;
+        return cpx<T>{-f * x.y, f * x.x};
+      } else if constexpr (KIND <= SYM_INV_LAPLACE) {
+        const int wk = nyq0(k, Nk);
+        const T wave = -(T)(wk * wk + wf0 * wf0 + wz0 * wz0);
+        T f;
+        if constexpr (KIND == SYM_LAPLACE) f = sc * wave;
+        else if constexpr (KIND == SYM_BIHARMONIC) f = (sc * wave) * wave;
+        else f = wave == (T)0 ? (T)0 : sc / wave;
+        return cpx<T>{f * x.x, f * x.y};
+      } else if constexpr (KIND == SYM_INV_BIHARMONIC) {
+        const int wk = plain(k, Nk);
+        const T wave = (T)(wk * wk + wf1 * wf1 + wz1 * wz1);
+        const T f = wave == (T)0 ? sc : (sc / wave) / wave;
+        return cpx<T>{f * x.x, f * x.y};
+      } else {
+        const long X = a.maxis == 0 ? (long)k : (long)u, Y = a.maxis == 0 ? (long)u : (long)k;
+        return apply_sym<T>(KIND, X, Y, (long)Z, a.N0, a.N1, a.N2, a.scale, x);
       }
     };
+    const int k1 = kc0 + kl;
+    auto inverse = [&](auto kind) {
+      auto pre = [&](int n, cpx<T> x) { return sym_elem(kind, k1 + R1 * n, x); };
+      static_fft<T, M, NT, +1, W, 0, 1>(acc, q, tw2, buf, 0, pre);
+    };
     if constexpr (MODE == M_DERIV) {
-      sym_loop(std::integral_constant<int, 0>{});
+      inverse(std::integral_constant<int, 0>{});
     } else {
       switch (a.sym) {
-        case SYM_GRAD_X: sym_loop(std::integral_constant<int, SYM_GRAD_X>{}); break;
-        case SYM_GRAD_Y: sym_loop(std::integral_constant<int, SYM_GRAD_Y>{}); break;
-        case SYM_GRAD_Z: sym_loop(std::integral_constant<int, SYM_GRAD_Z>{}); break;
-        case SYM_LAPLACE: sym_loop(std::integral_constant<int, SYM_LAPLACE>{}); break;
-        case SYM_BIHARMONIC: sym_loop(std::integral_constant<int, SYM_BIHARMONIC>{}); break;
-        case SYM_INV_LAPLACE: sym_loop(std::integral_constant<int, SYM_INV_LAPLACE>{}); break;
-        case SYM_INV_BIHARMONIC: sym_loop(std::integral_constant<int, SYM_INV_BIHARMONIC>{}); break;
-        case SYM_POISSON_DISCRETE: sym_loop(std::integral_constant<int, SYM_POISSON_DISCRETE>{}); break;
-        default: sym_loop(std::integral_constant<int, SYM_POISSON_CONTINUOUS>{}); break;
+        case SYM_GRAD_X: inverse(std::integral_constant<int, SYM_GRAD_X>{}); break;
+        case SYM_GRAD_Y: inverse(std::integral_constant<int, SYM_GRAD_Y>{}); break;
+        case SYM_GRAD_Z: inverse(std::integral_constant<int, SYM_GRAD_Z>{}); break;
+        case SYM_LAPLACE: inverse(std::integral_constant<int, SYM_LAPLACE>{}); break;
+        case SYM_BIHARMONIC: inverse(std::integral_constant<int, SYM_BIHARMONIC>{}); break;
+        case SYM_INV_LAPLACE: inverse(std::integral_constant<int, SYM_INV_LAPLACE>{}); break;
+        case SYM_INV_BIHARMONIC: inverse(std::integral_constant<int, SYM_INV_BIHARMONIC>{}); break;
+        case SYM_POISSON_DISCRETE: inverse(std::integral_constant<int, SYM_POISSON_DISCRETE>{}); break;
+        default: inverse(std::integral_constant<int, SYM_POISSON_CONTINUOUS>{}); break;
       }
     }
   }
-  __syncthreads();
-  static_fft<T, M, NT, +1, W, 0, 1>(acc, q, tw2, buf, 0);
 #pragma unroll
   for (int d = 0; d < D1; ++d)
 #pragma unroll

@@ -241,16 +241,22 @@ __global__ void __launch_bounds__(64) dft16_selftest_kernel(const cpx<float>* x,
 // acc/q: this thread's line and index within it (vector stages);
 // tile/LS: the whole tile (matrix-core stages).  Caller must
 // __syncthreads() before (data and twiddles staged).
-template <typename T, int N, int NT, int SIGN, int ES, int SPLIT, int LINES, int PAD = 0>
+// pre: element-wise operation on the input as the first stage loads it
+// (vector first stage only).
+template <typename T, int N, int NT, int SIGN, int ES, int SPLIT, int LINES, int PAD = 0,
+          typename PRE = NoPre>
 __device__ __forceinline__ void static_fft(const LineAcc<T, ES, PAD>& acc, int q,
-                                           const cpx<T>* __restrict__ tw, cpx<T>* tile, int LS) {
+                                           const cpx<T>* __restrict__ tw, cpx<T>* tile, int LS,
+                                           const PRE& pre = PRE{}) {
   using S = SchedOf<N, SPLIT>;
   constexpr int R0 = S::r(0), R1 = S::r(1), R2 = S::r(2);
   static_assert(!PAD || !split_sched(SPLIT), "matrix-core stages read the unpadded tile");
+  static_assert(std::is_same<PRE, NoPre>::value || !(split_sched(SPLIT) && R0 == 16),
+                "an input operation needs a vector first stage");
   if constexpr (split_sched(SPLIT) && R0 == 16)
     mstage<N, 1, SIGN, 0, ES, LINES, SPLIT>(tile, LS, tw);
   else
-    sstage<T, N, R0, 1, NT, SIGN, 0, ES>(acc, q, tw);
+    sstage<T, N, R0, 1, NT, SIGN, 0, ES, PAD, PRE>(acc, q, tw, pre);
   if constexpr (S::S > 1) {
     if constexpr (split_sched(SPLIT) && R1 == 16)
       mstage<N, sched_ns<N, SPLIT>(1), SIGN, sched_twoff<N, SPLIT>(1), ES, LINES, SPLIT>(tile, LS, tw);
