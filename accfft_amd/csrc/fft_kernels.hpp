@@ -1072,11 +1072,24 @@ AFFT_S4FCFG(float, 512, 16, 512, 8, 6)
 AFFT_S4FCFG(float, 1024, 8, 512, 8, 6)
 AFFT_S4FCFG(float, 2048, 4, 512, 16, 1)
 #undef AFFT_S4FCFG
+// M_MULT tiles where they differ from the M_DERIV ones: at 512 points
+// eight-column tiles of 256 threads, four workgroups per CU instead of two
+// (512^3 Poisson solve 1.102 -> 1.055 ms, laplace-family unchanged; the
+// derivative passes lose 8 % with them).
+template <typename T, int N>
+struct S4FMult : S4FCfg<T, N> {};
+template <>
+struct S4FMult<float, 512> {
+  static constexpr bool OK = true;
+  static constexpr int W = 8, TPB = 256, KC = 16, WPE = 1;
+};
+template <typename T, int N, int MODE>
+using S4FOf = typename std::conditional<MODE == M_MULT, S4FMult<T, N>, S4FCfg<T, N>>::type;
 
 template <typename T, int N, int MODE>
-__global__ void __launch_bounds__((S4FCfg<T, N>::TPB), (MODE == M_DERIV ? S4FCfg<T, N>::WPE : 1))
+__global__ void __launch_bounds__((S4FOf<T, N, MODE>::TPB), (MODE == M_DERIV ? S4FOf<T, N, MODE>::WPE : 1))
     sfft4f_kernel(FftArgs<T> a, const cpx<T>* __restrict__ twsub) {
-  using CF = S4FCfg<T, N>;
+  using CF = S4FOf<T, N, MODE>;
   // M_MULT keeps the whole tile in LDS (chunked, its symbol kinds need more
   // registers than two workgroups per CU allow)
   constexpr int R1 = 16, W = CF::W, TPB = CF::TPB, M = N / R1, KC = MODE == M_DERIV ? CF::KC : R1;

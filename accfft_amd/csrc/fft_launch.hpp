@@ -339,11 +339,17 @@ hipError_t launch_s4f(int mode, FftArgs<T> a, const AxisTab<T>& tab, hipStream_t
 #define AFFT_S4F(NN)                                                                        \
   if (tab.n == NN) {                                                                        \
     if constexpr (S4FCfg<T, NN>::OK) {                                                      \
-      using CF = S4FCfg<T, NN>;                                                             \
-      a.lpw = CF::W;                                                                        \
-      dim3 g((unsigned)((a.nlines + CF::W - 1) / CF::W)), b(CF::TPB);                       \
-      if (mode == M_MULT) hipLaunchKernelGGL((sfft4f_kernel<T, NN, M_MULT>), g, b, 0, st, a, tab.tw_sub); \
-      else hipLaunchKernelGGL((sfft4f_kernel<T, NN, M_DERIV>), g, b, 0, st, a, tab.tw_sub); \
+      if (mode == M_MULT) {                                                                 \
+        using CF = S4FOf<T, NN, M_MULT>;                                                    \
+        a.lpw = CF::W;                                                                      \
+        dim3 g((unsigned)((a.nlines + CF::W - 1) / CF::W)), b(CF::TPB);                     \
+        hipLaunchKernelGGL((sfft4f_kernel<T, NN, M_MULT>), g, b, 0, st, a, tab.tw_sub);    \
+      } else {                                                                              \
+        using CF = S4FOf<T, NN, M_DERIV>;                                                   \
+        a.lpw = CF::W;                                                                      \
+        dim3 g((unsigned)((a.nlines + CF::W - 1) / CF::W)), b(CF::TPB);                     \
+        hipLaunchKernelGGL((sfft4f_kernel<T, NN, M_DERIV>), g, b, 0, st, a, tab.tw_sub);   \
+      }                                                                                     \
       return hipGetLastError();                                                             \
     }                                                                                       \
   }
