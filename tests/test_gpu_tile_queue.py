@@ -64,3 +64,29 @@ def test_repeated_executions_match_oracle(dev, orc, shape, variant):
         else:
             assert np.array_equal(got, firstb), rep
     plan.destroy()
+
+
+def test_two_plans_on_two_streams(dev, orc):
+    """Each plan has its own queue: two plans of the same shape run their
+    queued x passes on two streams at once and both match the oracle."""
+    import torch
+    import accfft_amd as af
+    shape = (2048, 40, 254)
+    print(f"tile queue two streams shape={shape}", flush=True)
+    xs = [rng(900 + i).standard_normal(shape) for i in range(2)]
+    a = [torch.tensor(x, dtype=torch.float32, device=dev) for x in xs]
+    h = [torch.empty((shape[0], shape[1], shape[2] // 2 + 1), dtype=torch.complex64, device=dev)
+         for _ in range(2)]
+    plans = [af.Plan("r2c", shape, a[i], h[i], af.F32, af.ACCFFT_ESTIMATE) for i in range(2)]
+    streams = [torch.cuda.Stream() for _ in range(2)]
+    for s in streams:
+        s.wait_stream(torch.cuda.current_stream())
+    for rep in range(3):
+        for i in range(2):
+            with torch.cuda.stream(streams[i]):
+                plans[i].execute_r2c(a[i], h[i])
+    torch.cuda.synchronize()
+    for i in range(2):
+        assert rel_l2(h[i].cpu().numpy(), orc.r2c(xs[i])) < 1e-5, i
+    for p in plans:
+        p.destroy()
