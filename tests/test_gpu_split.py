@@ -60,7 +60,8 @@ def _split_plan(kind, n, a, b, split):
 
 
 @pytest.mark.parametrize("split", list(SPLITS))
-@pytest.mark.parametrize("shape", [(256, 256, 256), (64, 512, 1024), (2048, 16, 256), (16, 4096, 8)])
+@pytest.mark.parametrize("shape", [(256, 256, 256), (64, 512, 1024), (2048, 16, 256), (16, 4096, 8),
+                                   (8, 32, 512), (4, 8, 2048), (2, 4, 4096)])
 def test_split_r2c_c2r(dev, orc, split, shape):
     import torch
     print(f"split {split} r2c/c2r {shape}", flush=True)
