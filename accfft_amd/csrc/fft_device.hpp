@@ -414,7 +414,7 @@ __device__ __forceinline__ void dstage(const cpx<T>* src, cpx<T>* dst, int N, in
     for (int r = 0; r < R; ++r) {
       cpx<T> x = src[lay.addr(line, j + r * NB)];
       if (r > 0 && NS > 1) {
-        const cpx<T> w = twg[(r * k * tstep) % N];
+        const cpx<T> w = twg[r * k * tstep];  // r k tstep < R NS tstep = N
         x = (SIGN < 0) ? cmul(x, w) : cmulc(x, w);
       }
       v[r] = x;

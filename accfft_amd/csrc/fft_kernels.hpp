@@ -90,6 +90,8 @@ __device__ __forceinline__ int64_t row_off(int r, int64_t S, int shift, int64_t 
 }
 
 // -------------------------------------------------------------- configs
+// Global loads each thread of a dynamic-engine kernel keeps in flight.
+constexpr int DYN_BATCH = 8;
 template <int N, int SPLIT = 0, int ZT = 256>
 struct ZCfg {
   static constexpr int NT = sched_minnb<N, SPLIT>();
@@ -320,39 +322,47 @@ __global__ void __launch_bounds__(ZT > 256 ? ZT : 256) zfft_kernel(FftArgs<T> a)
 #pragma unroll
     for (int j = 0; j < EP; ++j) bufA[P(t + j * TPB)] = v[j];
     }
-  } else if constexpr (MODE == M_C2C || MODE == M_MULT) {
-    for (int e = t; e < lpw * n; e += TPB) {
-      const int p = e / n, i = e - p * n;
-      const int64_t g = l0 + p;
-      cpx<T> v = {(T)0, (T)0};
-      if (g < nl) v = ldc<T>(a.in, g * a.idist + i);
-      bufA[e] = v;
-    }
-  } else if constexpr (MODE == M_R2C || MODE == M_DERIV) {
-    for (int e = t; e < lpw * n; e += TPB) {
-      const int p = e / n, i = e - p * n;
-      const int64_t ra = 2 * (l0 + p), rb = ra + 1;
-      cpx<T> v = {(T)0, (T)0};
-      if (ra < nl) v.x = ldr<T>(a.in, ra * a.idist + i);
-      if (rb < nl) v.y = ldr<T>(a.in, rb * a.idist + i);
-      bufA[e] = v;
-    }
-  } else {  // M_C2R
-    for (int e = t; e < lpw * n; e += TPB) {
-      const int p = e / n, i = e - p * n;
-      const int64_t ga = 2 * (l0 + p), gb = ga + 1;
-      const int k = i < nh ? i : n - i;
-      cpx<T> A = {(T)0, (T)0}, B = {(T)0, (T)0};
-      if (ga < nl) A = ldc<T>(a.in, zcol_off(a, ga, a.idist, k, nh));
-      if (gb < nl) B = ldc<T>(a.in, zcol_off(a, gb, a.idist, k, nh));
-      if (k == 0 || 2 * k == n) {
-        A.y = (T)0;
-        B.y = (T)0;
-      } else if (a.adjw) {
-        A = cscale(A, (T)0.5);
-        B = cscale(B, (T)0.5);
+  } else {
+    // dynamic engine: batches of DYN_BATCH loads per thread in flight
+    // before their LDS writes (a flat load cannot move past an LDS store,
+    // so one load per iteration would wait out the memory latency each time)
+    const int ne = lpw * n;
+    for (int e0 = t; e0 < ne; e0 += DYN_BATCH * TPB) {
+      cpx<T> v[DYN_BATCH];
+#pragma unroll
+      for (int j = 0; j < DYN_BATCH; ++j) {
+        const int e = e0 + j * TPB;
+        const int p = e / n, i = e - p * n;
+        cpx<T> z = {(T)0, (T)0};
+        if constexpr (MODE == M_C2C || MODE == M_MULT) {
+          const int64_t g = l0 + p;
+          if (e < ne && g < nl) z = ldc<T>(a.in, g * a.idist + i);
+        } else if constexpr (MODE == M_R2C || MODE == M_DERIV) {
+          const int64_t ra = 2 * (l0 + p), rb = ra + 1;
+          if (e < ne && ra < nl) z.x = ldr<T>(a.in, ra * a.idist + i);
+          if (e < ne && rb < nl) z.y = ldr<T>(a.in, rb * a.idist + i);
+        } else {  // M_C2R
+          const int64_t ga = 2 * (l0 + p), gb = ga + 1;
+          const int k = i < nh ? i : n - i;
+          cpx<T> A = {(T)0, (T)0}, B = {(T)0, (T)0};
+          if (e < ne && ga < nl) A = ldc<T>(a.in, zcol_off(a, ga, a.idist, k, nh));
+          if (e < ne && gb < nl) B = ldc<T>(a.in, zcol_off(a, gb, a.idist, k, nh));
+          if (k == 0 || 2 * k == n) {
+            A.y = (T)0;
+            B.y = (T)0;
+          } else if (a.adjw) {
+            A = cscale(A, (T)0.5);
+            B = cscale(B, (T)0.5);
+          }
+          z = i < nh ? cpx<T>{A.x - B.y, A.y + B.x} : cpx<T>{A.x + B.y, B.x - A.y};
+        }
+        v[j] = z;
       }
-      bufA[e] = i < nh ? cpx<T>{A.x - B.y, A.y + B.x} : cpx<T>{A.x + B.y, B.x - A.y};
+#pragma unroll
+      for (int j = 0; j < DYN_BATCH; ++j) {
+        const int e = e0 + j * TPB;
+        if (e < ne) bufA[e] = v[j];
+      }
     }
   }
   __syncthreads();
@@ -549,30 +559,37 @@ __global__ void __launch_bounds__(256) sfft_kernel(FftArgs<T> a) {
     stage_twiddles<T, TWN, TPB>(const_cast<cpx<T>*>(twl), a.tw, t);
 #pragma unroll
     for (int j = 0; j < RP; ++j) bufA[(r0 + j * RS) * WS + w] = v[j];
-  } else if constexpr (MODE == M_C2C || MODE == M_MULT) {
-    for (int r = r0; r < n; r += rstep) {
-      cpx<T> v = {(T)0, (T)0};
-      if (va) v = ldc<T>(a.in, ia + row_off(r, a.Sin, a.ishift, a.Bin));
-      bufA[r * W + w] = v;
-    }
-  } else if constexpr (MODE == M_R2C || MODE == M_DERIV) {
-    for (int r = r0; r < n; r += rstep) {
-      cpx<T> v = {(T)0, (T)0};
-      if (va) v.x = ldr<T>(a.in, ia + (int64_t)r * a.Sin);
-      if (vb) v.y = ldr<T>(a.in, ib + (int64_t)r * a.Sin);
-      bufA[r * W + w] = v;
-    }
-  } else {  // M_C2R
-    for (int r = r0; r < n; r += rstep) {
-      const int k = r < nh ? r : n - r;
-      cpx<T> A = {(T)0, (T)0}, B = {(T)0, (T)0};
-      if (va) A = ldc<T>(a.in, ia + (int64_t)k * a.Sin);
-      if (vb) B = ldc<T>(a.in, ib + (int64_t)k * a.Sin);
-      if (k == 0 || 2 * k == n) { A.y = (T)0; B.y = (T)0; }
-      cpx<T> z;
-      if (r < nh) z = {A.x - B.y, A.y + B.x};
-      else z = {A.x + B.y, B.x - A.y};
-      bufA[r * W + w] = z;
+  } else {
+    // dynamic engine: batches of DYN_BATCH row loads in flight per thread
+    // before their LDS writes (as in the z kernel)
+    for (int rb0 = r0; rb0 < n; rb0 += DYN_BATCH * rstep) {
+      cpx<T> v[DYN_BATCH];
+#pragma unroll
+      for (int j = 0; j < DYN_BATCH; ++j) {
+        const int r = rb0 + j * rstep;
+        const bool in = r < n;
+        cpx<T> z = {(T)0, (T)0};
+        if constexpr (MODE == M_C2C || MODE == M_MULT) {
+          if (in && va) z = ldc<T>(a.in, ia + row_off(r, a.Sin, a.ishift, a.Bin));
+        } else if constexpr (MODE == M_R2C || MODE == M_DERIV) {
+          if (in && va) z.x = ldr<T>(a.in, ia + (int64_t)r * a.Sin);
+          if (in && vb) z.y = ldr<T>(a.in, ib + (int64_t)r * a.Sin);
+        } else {  // M_C2R
+          const int k = r < nh ? r : n - r;
+          cpx<T> A = {(T)0, (T)0}, B = {(T)0, (T)0};
+          if (in && va) A = ldc<T>(a.in, ia + (int64_t)k * a.Sin);
+          if (in && vb) B = ldc<T>(a.in, ib + (int64_t)k * a.Sin);
+          if (k == 0 || 2 * k == n) { A.y = (T)0; B.y = (T)0; }
+          if (r < nh) z = {A.x - B.y, A.y + B.x};
+          else z = {A.x + B.y, B.x - A.y};
+        }
+        v[j] = z;
+      }
+#pragma unroll
+      for (int j = 0; j < DYN_BATCH; ++j) {
+        const int r = rb0 + j * rstep;
+        if (r < n) bufA[r * W + w] = v[j];
+      }
     }
   }
   __syncthreads();

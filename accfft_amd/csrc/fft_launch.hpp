@@ -70,7 +70,7 @@ inline bool static_length(int n) {
 inline bool pair_mode(int mode) { return mode == M_R2C || mode == M_C2R || mode == M_DERIV; }
 
 inline int dyn_lines(int n) {
-  int l = 4096 / n;
+  int l = 2048 / n;
   if (l < 1) l = 1;
   if (l > 64) l = 64;
   return l;
