@@ -212,11 +212,46 @@ __device__ __forceinline__ void dft_odd(cpx<T>* v) {
   }
 }
 
+// Radix 9 as 3 x 3 (n = 3 n1 + n2, k = k1 + 3 k2): DFT-3 over n1 for each
+// n2, twiddles W9^(n2 k1), DFT-3 over n2 for each k1.
+template <typename T, int SIGN>
+__device__ __forceinline__ void dft9(cpx<T>* v) {
+  cpx<T> y[3][3];
+#pragma unroll
+  for (int n2 = 0; n2 < 3; ++n2) {
+    cpx<T> t[3] = {v[n2], v[3 + n2], v[6 + n2]};
+    dft_odd<T, 3, SIGN>(t);
+#pragma unroll
+    for (int k1 = 0; k1 < 3; ++k1) y[n2][k1] = t[k1];
+  }
+  // W9^m = cos(2 pi m/9) + SIGN i sin(2 pi m/9), m = 1, 2, 4
+  const T c1 = (T)0.76604444311897803520, s1 = (T)0.64278760968653932632;
+  const T c2 = (T)0.17364817766693034885, s2 = (T)0.98480775301220805936;
+  const T c4 = (T)-0.93969262078590838405, s4 = (T)0.34202014332566873304;
+  auto rot = [](cpx<T> x, T c, T s) {
+    const T si = SIGN * s;
+    return cpx<T>{x.x * c - x.y * si, x.x * si + x.y * c};
+  };
+  y[1][1] = rot(y[1][1], c1, s1);
+  y[1][2] = rot(y[1][2], c2, s2);
+  y[2][1] = rot(y[2][1], c2, s2);
+  y[2][2] = rot(y[2][2], c4, s4);
+#pragma unroll
+  for (int k1 = 0; k1 < 3; ++k1) {
+    cpx<T> t[3] = {y[0][k1], y[1][k1], y[2][k1]};
+    dft_odd<T, 3, SIGN>(t);
+    v[k1] = t[0];
+    v[k1 + 3] = t[1];
+    v[k1 + 6] = t[2];
+  }
+}
+
 template <typename T, int R, int SIGN>
 __device__ __forceinline__ void dft(cpx<T>* v) {
   if constexpr (R == 2) dft2<T, SIGN>(v);
   else if constexpr (R == 4) dft4<T, SIGN>(v);
   else if constexpr (R == 8) dft8<T, SIGN>(v);
+  else if constexpr (R == 9) dft9<T, SIGN>(v);
   else if constexpr (R == 16) dft16<T, SIGN>(v);
   else dft_odd<T, R, SIGN>(v);
 }
@@ -471,6 +506,7 @@ __device__ cpx<T>* dyn_fft(cpx<T>* a, cpx<T>* b, const DynSched& sc, int nlines,
       case 5: dstage<T, 5, SIGN>(a, b, N, NS, nlines, lay, twg); break;
       case 7: dstage<T, 7, SIGN>(a, b, N, NS, nlines, lay, twg); break;
       case 8: dstage<T, 8, SIGN>(a, b, N, NS, nlines, lay, twg); break;
+      case 9: dstage<T, 9, SIGN>(a, b, N, NS, nlines, lay, twg); break;
       case 16: dstage<T, 16, SIGN>(a, b, N, NS, nlines, lay, twg); break;
       default: dstage_generic<T, SIGN>(a, b, N, R, NS, nlines, lay, twg); break;
     }

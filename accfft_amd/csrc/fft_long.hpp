@@ -3,7 +3,7 @@
 // FP64 > 5120 points that are not powers of two up to 4096).
 //
 // Each radix stage of the line's mixed-radix Stockham schedule (the dynamic
-// engine's schedule: radices 16, 8, 4, 2, 3, 5, 7, then any other prime)
+// engine's schedule: radices 16, 8, 4, 2, 9, 3, 5, 7, then any other prime)
 // is one pass over HBM: butterfly j of stage (radix R, span NS) reads
 // element j + r N/R of every line, multiplies by exp(-+2 pi i r (j mod NS)
 // / (NS R)) from the axis's length-N twiddle table, does the R-point DFT
@@ -287,6 +287,7 @@ hipError_t launch_gstage(const cpx<T>* src, GGeom gs, cpx<T>* dst, GGeom gd, int
     AFFT_GST(5)
     AFFT_GST(7)
     AFFT_GST(8)
+    AFFT_GST(9)
     AFFT_GST(16)
 #undef AFFT_GST
     default:

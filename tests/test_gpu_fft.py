@@ -211,3 +211,43 @@ def test_fused_z_y_first_level(dev, orc, shape, prec):
     plan.set_tuning(1, -1)
     assert plan.tuning()[1] != -3
     plan.destroy()
+
+
+# Powers of three: radix-9 stages of the run-time schedule (729 = 9 9 9,
+# 243 = 9 9 3) in the LDS tiles of every axis, and in the global radix passes
+# of lines longer than one tile (19683 = 9^4 3 in both precisions, 6561 =
+# 9^4 in FP64).  The oracle factors its lines with radix 3 only.
+P3_SHAPES = [(27, 81, 243), (9, 729, 18), (729, 3, 9), (4, 3, 2187), (2, 3, 19683), (6561, 2, 4)]
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("shape", P3_SHAPES)
+def test_powers_of_three(dev, orc, shape, prec):
+    import torch
+    import accfft_amd as af
+    print(f"powers of three shape={shape} prec={prec}", flush=True)
+    n = list(shape)
+    fp = af.F64 if prec == "f64" else af.F32
+    x = rng(sum(shape)).standard_normal(shape)
+    xd = _t(dev, x, prec)
+    h = torch.empty((n[0], n[1], n[2] // 2 + 1),
+                    dtype=torch.complex128 if prec == "f64" else torch.complex64, device=dev)
+    plan = af.Plan("r2c", n, xd, h, fp)
+    plan.execute_r2c(xd, h)
+    torch.cuda.synchronize()
+    ref = orc.r2c(x)
+    assert rel_l2(h.cpu().numpy(), ref) < TOL[prec]
+    back = torch.empty_like(xd)
+    plan.execute_c2r(h, back)
+    torch.cuda.synchronize()
+    assert rel_l2(back.cpu().numpy() / np.prod(n), x) < TOL[prec]
+    plan.destroy()
+    r = rng(sum(shape) + 1)
+    z = r.standard_normal(shape) + 1j * r.standard_normal(shape)
+    zd = _tc(dev, z, prec)
+    out = torch.empty_like(zd)
+    pc = af.Plan("c2c", n, zd, out, fp)
+    pc.execute_c2c(af.ACCFFT_BACKWARD, zd, out)
+    torch.cuda.synchronize()
+    assert rel_l2(out.cpu().numpy(), orc.c2c(z, +1)) < TOL[prec]
+    pc.destroy()
