@@ -416,20 +416,36 @@ struct DynSched {
 // Flat butterfly index b -> (line, j).  Layout Z: line stride LS, element
 // stride 1, threads walk j fastest.  Layout S: element stride W, line (= the
 // column inside the tile) fastest.
+// q = b / d with r = b % d for 0 <= b < 2^22 and d >= 1, given rcp =
+// 1.0f / d: the float quotient is within one of the true one (relative
+// error below 2^-23), so one correction step makes it exact.  A few VALU
+// operations instead of the integer division sequence.
+__device__ __forceinline__ int fdivmod(int b, int d, float rcp, int& r) {
+  int q = (int)((float)b * rcp);
+  r = b - q * d;
+  if (r < 0) {
+    --q;
+    r += d;
+  } else if (r >= d) {
+    ++q;
+    r -= d;
+  }
+  return q;
+}
+
 struct LayZ {
   int ls;
   __device__ __forceinline__ int addr(int line, int n) const { return line * ls + n; }
-  __device__ __forceinline__ void map(int b, int nb, int& line, int& j) const {
-    line = b / nb;
-    j = b - line * nb;
+  __device__ __forceinline__ void map(int b, int nb, float rnb, int& line, int& j) const {
+    line = fdivmod(b, nb, rnb, j);
   }
 };
 struct LayS {
   int w;
   __device__ __forceinline__ int addr(int line, int n) const { return n * w + line; }
-  __device__ __forceinline__ void map(int b, int nb, int& line, int& j) const {
-    j = b / w;
-    line = b - j * w;
+  __device__ __forceinline__ void map(int b, int nb, float rnb, int& line, int& j) const {
+    j = b >> (__ffs(w) - 1);  // w is a power of two
+    line = b & (w - 1);
   }
 };
 
@@ -440,10 +456,11 @@ __device__ __forceinline__ void dstage(const cpx<T>* src, cpx<T>* dst, int N, in
   const int NB = N / R;
   const int total = nlines * NB;
   const int tstep = N / (NS * R);
+  const float rnb = 1.0f / (float)NB, rns = 1.0f / (float)NS;
   for (int b = threadIdx.x; b < total; b += blockDim.x) {
-    int line, j;
-    lay.map(b, NB, line, j);
-    const int k = j % NS;
+    int line, j, k;
+    lay.map(b, NB, rnb, line, j);
+    (void)fdivmod(j, NS, rns, k);
     cpx<T> v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -470,10 +487,11 @@ __device__ __forceinline__ void dstage_generic(const cpx<T>* src, cpx<T>* dst, i
   const int total = nlines * NB;
   const int tstep = N / (NS * R);
   const int rstep = N / R;
+  const float rnb = 1.0f / (float)NB, rns = 1.0f / (float)NS;
   for (int b = threadIdx.x; b < total; b += blockDim.x) {
-    int line, j;
-    lay.map(b, NB, line, j);
-    const int k = j % NS;
+    int line, j, k;
+    lay.map(b, NB, rnb, line, j);
+    (void)fdivmod(j, NS, rns, k);
     const int base = (j - k) * R + k;
     for (int q = 0; q < R; ++q) {
       cpx<T> acc = {(T)0, (T)0};
