@@ -485,11 +485,17 @@ __global__ void __launch_bounds__(ZT > 256 ? ZT : 256) zfft_kernel(FftArgs<T> a)
 // ====================================================================
 // Strided lines (y, x)
 // ====================================================================
+// Threads per block of the dynamic-engine strided kernel: 512 in FP32 (twice
+// the butterflies in flight per tile: 1000^3 round trip 28.1 -> 24.6 ms,
+// 729^3 9.4 -> 8.3 ms), 256 in FP64 (512 threads: 768^3 18.6 -> 22.0 ms).
+template <typename T>
+constexpr int dyn_s_tpb() { return sizeof(T) == 4 ? 512 : 256; }
+
 template <typename T, int N, int MODE, int SIGN, int SPLIT = 0, int LDSK = 64>
-__global__ void __launch_bounds__(256) sfft_kernel(FftArgs<T> a) {
+__global__ void __launch_bounds__(N > 0 ? 256 : dyn_s_tpb<T>()) sfft_kernel(FftArgs<T> a) {
   constexpr bool STATIC = N > 0;
   constexpr int NS_ = STATIC ? N : 16;
-  constexpr int TPB = 256;
+  constexpr int TPB = STATIC ? 256 : dyn_s_tpb<T>();
   constexpr int WS = SCfg<T, NS_, SPLIT, LDSK>::W;
   constexpr int TWN = STATIC ? sched_twn<NS_, SPLIT>() : 0;
   const int n = STATIC ? N : a.sc.n;

@@ -107,7 +107,7 @@ hipError_t launch_z_n(int mode, int sign, FftArgs<T>& a, int64_t grid, size_t sh
 template <typename T, int N, int LDSK = 64>
 hipError_t launch_s_n(int mode, int sign, FftArgs<T>& a, int64_t grid, size_t shm,
                       hipStream_t st) {
-  dim3 g((unsigned)grid), b(256);
+  dim3 g((unsigned)grid), b(N > 0 ? 256 : dyn_s_tpb<T>());
   switch (mode) {
     case M_C2C:
       if (sign < 0) hipLaunchKernelGGL((sfft_kernel<T, N, M_C2C, -1, 0, LDSK>), g, b, shm, st, a);
